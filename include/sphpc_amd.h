@@ -1,0 +1,205 @@
+/*
+ * sphpc_amd -- MI355X (gfx950) position-based-fluids substep behind a C ABI.
+ *
+ * Drop-in GPU path for the single-rank solver of AdamSimpson/SPHPC.  The
+ * reference's driver (Simulation/src/simulation.c:54-116) calls, per substep,
+ * the stage functions of particles.h:44-92 and neighbors.h:35-47 on struct
+ * pointers, and those drop to native code through thrust_c.h:47-69 and
+ * obstacle_cuda.h:21-29.  This header exports the same stages, in the same
+ * order and with the same meaning, on an opaque context that owns FP32 device
+ * buffers, plus the native sort/bounds/collision entry points.
+ *
+ * Conventions
+ *  - Every function returns 0 (SPH_OK) or a negative status; sph_last_error()
+ *    gives the message.  The reference aborts on fatal errors (EXIT_PRINT,
+ *    Common/print_macros.h:14-16); the bundled driver keeps that behaviour by
+ *    exiting on a non-zero status.
+ *  - Positions, velocities and all per-particle fields are FP32 on the device.
+ *    Host<->device copies happen only in upload/download calls.
+ *  - All work of one context runs in order on the context's own HIP stream;
+ *    calls on one context must not overlap from several host threads.
+ *  - The boundary box must start at the origin (the reference hashes absolute
+ *    coordinates, neighbors.c:86-95, and tests x < 0, neighbors.c:257-260).
+ */
+#ifndef SPHPC_AMD_H_
+#define SPHPC_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPH_MAX_NEIGHBORS 60 /* neighbors.h:6 */
+
+enum sph_status {
+  SPH_OK = 0,
+  SPH_ERR_ARG = -1,      /* invalid argument */
+  SPH_ERR_HIP = -2,      /* HIP runtime error (message names the call) */
+  SPH_ERR_STATE = -3,    /* stage called out of order (e.g. no neighbour lists) */
+  SPH_ERR_CAPACITY = -4, /* more particles than the context was created for */
+  SPH_ERR_NOMEM = -5     /* device allocation failed */
+};
+
+/* Run parameters.  Fields of struct Params (simulation.h:6-21) plus the
+ * constants the reference hard-codes; sph_default_params() returns the
+ * reference's values. */
+typedef struct sph_params {
+  double smoothing_radius;  /* h; simulation.c:152 (2*float(spacing)) */
+  double time_step;         /* dt; ini SimParameters.time_step */
+  double g;                 /* gravity magnitude; ini PhysicalParameters.g */
+  double k;                 /* surface tension gamma; ini PhysicalParameters.k */
+  double c;                 /* XSPH viscosity; ini PhysicalParameters.c */
+  double rest_density;      /* particles.c:1034 (1000) */
+  double rest_mass;         /* particles.c:1035-1037 */
+  double lambda_epsilon;    /* relaxation, particles.c:663 (0.01) */
+  double vorticity_epsilon; /* confinement, particles.c:289 (0.1*h) */
+  double v_max;             /* velocity clamp, particles.c:791 (100) */
+  double stiffness_k;       /* stiffness schedule, particles.c:693 (0.95) */
+  double density_tolerance; /* 0: fixed iterations (reference); >0: stop when
+                               max_i max(rho_i-rho0,0)/rho0 <= tolerance */
+  int solve_iterations;     /* Jacobi iterations (cap), simulation.c:72 (10) */
+  int surface_tension;      /* run ComputeSurfaceTension (reference: 1) */
+  int viscosity;            /* run ApplyViscosity (reference: 1) */
+  int vorticity;            /* run ComputeVorticity+Confinement (reference: 1) */
+  int vorticity_model;      /* 0: reference formula (particles.c:207-381);
+                               1: omega weighted by m/rho_j, dv = dt*eps*(N x omega) */
+} sph_params;
+
+/* struct AABB, geometry.h:9-16 */
+typedef struct sph_aabb {
+  double min_x, max_x, min_y, max_y, min_z, max_z;
+} sph_aabb;
+
+/* Signed-distance obstacle (struct Obstacle, obstacle.h:8-29): an SDFGen grid
+ * (x fastest) of dim_x*dim_y*dim_z FP32 values in model units with texel size
+ * `spacing`, mapped onto the world box `world_bounds`. */
+typedef struct sph_obstacle_desc {
+  int dim_x, dim_y, dim_z;
+  double spacing;
+  sph_aabb world_bounds;
+  const float *sdf; /* host pointer */
+} sph_obstacle_desc;
+
+/* Per-substep counters read back by sph_get_stats. */
+typedef struct sph_stats {
+  int particles;
+  int iterations;            /* Jacobi updates applied in the last substep */
+  int converged;             /* tolerance mode: reached the tolerance */
+  float max_density_error;   /* last computed max_i max(rho_i-rho0,0)/rho0 */
+  unsigned long long neighbor_overflow; /* hits dropped by the 60 cap (silent in
+                                           the reference, neighbors.c:285-286) */
+  unsigned long long cell_clamps;       /* cell coordinates clamped into the grid */
+  long long substeps;        /* substeps run on this context */
+} sph_stats;
+
+typedef struct sph_ctx sph_ctx;
+
+const char *sph_last_error(void);
+int sph_get_device_count(int *count);
+
+/* Reference defaults for a given smoothing radius (simulation.c:130-178,
+ * particles.c:289,663,691-694,791). */
+void sph_default_params(sph_params *p, double smoothing_radius);
+
+/* --- lifecycle: AllocInitParticles/AllocInitNeighbors/AllocInitObstacle
+ *     (particles.h:104-108, neighbors.h:35-40, obstacle.h:32-33) --------- */
+int sph_create(const sph_params *params, const sph_aabb *boundary, int max_particles,
+               sph_ctx **out);
+int sph_destroy(sph_ctx *ctx);  /* FinalizeParticles/FinalizeNeighbors */
+int sph_set_params(sph_ctx *ctx, const sph_params *params);
+int sph_get_params(sph_ctx *ctx, sph_params *params);
+int sph_grid_dims(sph_ctx *ctx, int *nx, int *ny, int *nz);
+/* AllocInitObstacle_CUDA (obstacle_cuda.cu:133-183); sdf == NULL removes it */
+int sph_set_obstacle(sph_ctx *ctx, const sph_obstacle_desc *obstacle);
+/* Host -> device state (the acc enter data of particles.c:1056-1078). */
+int sph_upload_state(sph_ctx *ctx, int n, const float *x, const float *y, const float *z,
+                     const float *vx, const float *vy, const float *vz);
+/* Device -> host state in particle-id order (acc update host, fileio_adios.c:73-76). */
+int sph_download_state(sph_ctx *ctx, float *x, float *y, float *z, float *vx, float *vy,
+                       float *vz);
+int sph_particle_count(sph_ctx *ctx, int *n);
+
+/* --- per-substep stages, in the driver's order (simulation.c:59-108) ---- */
+int sph_apply_gravity(sph_ctx *ctx);               /* particles.c:516 */
+int sph_predict_positions(sph_ctx *ctx);           /* particles.c:748 */
+int sph_find_all_neighbors(sph_ctx *ctx);          /* neighbors.c:68 */
+int sph_compute_densities(sph_ctx *ctx);           /* particles.c:461 */
+int sph_compute_lambda(sph_ctx *ctx);              /* particles.c:588 */
+int sph_update_dps(sph_ctx *ctx, int substep);     /* particles.c:668 */
+int sph_update_position_stars(sph_ctx *ctx);       /* particles.c:529 */
+int sph_update_velocities(sph_ctx *ctx);           /* particles.c:787 */
+int sph_compute_surface_tension(sph_ctx *ctx);     /* particles.c:67 */
+int sph_apply_viscosity(sph_ctx *ctx);             /* particles.c:383 */
+int sph_compute_vorticity(sph_ctx *ctx);           /* particles.c:207 */
+int sph_apply_vorticity_confinement(sph_ctx *ctx); /* particles.c:285 */
+int sph_update_positions(sph_ctx *ctx);            /* particles.c:566 */
+
+/* Whole substeps with fused kernels: the loop body of simulation.c:54-116
+ * (output excluded), `substeps` times. */
+int sph_step(sph_ctx *ctx, int substeps);
+int sph_synchronize(sph_ctx *ctx);
+int sph_get_stats(sph_ctx *ctx, sph_stats *stats);
+/* FP64 reductions: kinetic energy sum(m/2 |v|^2) and max density error. */
+int sph_compute_energy(sph_ctx *ctx, double *kinetic_energy);
+
+/* --- inspection (particle-id order; valid after sph_find_all_neighbors) -- */
+enum sph_field {
+  SPH_FIELD_POSITION_STAR = 0, /* x* (3 floats per particle) */
+  SPH_FIELD_DENSITY = 1,       /* rho (1) */
+  SPH_FIELD_LAMBDA = 2,        /* lambda (1) */
+  SPH_FIELD_DP = 3,            /* delta p of sph_update_dps (3) */
+  SPH_FIELD_VELOCITY = 4,      /* v in the velocity passes (3) */
+  SPH_FIELD_COLOR = 5,         /* colour-field normal (3) */
+  SPH_FIELD_VORTICITY = 6,     /* omega (3) */
+  SPH_FIELD_POSITION_OLD = 7   /* x at substep start (3) */
+};
+int sph_download_field(sph_ctx *ctx, int field, float *out);
+/* Neighbour lists as particle ids in the reference order (NeighborBucket,
+ * neighbors.h:25-28): ids[i*60 + j], j < counts[i]. */
+int sph_download_neighbors(sph_ctx *ctx, uint32_t *ids, int *counts);
+/* Sorted (cell key, particle id) pairs of the last neighbour search. */
+int sph_download_sorted(sph_ctx *ctx, uint32_t *keys, uint32_t *ids);
+/* Cell tables: start[c] = lower bound, end[c] = upper bound (cells entries). */
+int sph_download_cell_bounds(sph_ctx *ctx, uint32_t *start, uint32_t *end);
+
+/* --- native sub-boundary (device pointers; stream NULL = default) ------- */
+/* SortByKey, thrust_c.h:47-50: stable sort of n (key, value) pairs by key. */
+int sph_sort_by_key(uint32_t *keys, uint32_t *values, int n, void *stream);
+/* Same, sorting only the low `key_bits` bits (keys must be < 2^key_bits). */
+int sph_sort_by_key_bits(uint32_t *keys, uint32_t *values, int n, int key_bits,
+                         void *stream);
+/* FindLowerBounds/FindUpperBounds, thrust_c.h:52-56,65-69: for every cell
+ * c in [0, cells) the lower/upper bound of c in n sorted keys. */
+int sph_find_lower_bounds(const uint32_t *keys, int n, int cells, uint32_t *out,
+                          void *stream);
+int sph_find_upper_bounds(const uint32_t *keys, int n, int cells, uint32_t *out,
+                          void *stream);
+/* CalculateParticleCollisions, obstacle_cuda.h:21-29: push particles inside
+ * the obstacle out along the SDF gradient.  The obstacle is an uploaded grid. */
+typedef struct sph_sdf sph_sdf;
+int sph_sdf_create(const sph_obstacle_desc *desc, sph_sdf **out);
+int sph_sdf_destroy(sph_sdf *sdf);
+int sph_calculate_particle_collisions(float *x, float *y, float *z, int n, const sph_sdf *sdf,
+                                      void *stream);
+
+/* Device memory plumbing for callers that hold raw device pointers. */
+int sph_device_malloc(void **ptr, size_t bytes);
+int sph_device_free(void *ptr);
+int sph_memcpy_htod(void *dst, const void *src, size_t bytes);
+int sph_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+
+/* --- per-kernel timing (HIP events on the context stream) -------------- */
+int sph_profile_enable(sph_ctx *ctx, int enable);
+/* Sum of event-timed durations (ms) and launch count of one kernel since the
+ * last reset; kernel names as in sph_profile_kernel_name. */
+int sph_profile_read(sph_ctx *ctx, int kernel, double *total_ms, long long *launches);
+int sph_profile_reset(sph_ctx *ctx);
+const char *sph_profile_kernel_name(int kernel);
+int sph_profile_kernel_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

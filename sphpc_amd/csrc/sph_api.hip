@@ -1,0 +1,798 @@
+// C ABI of libsphpc_amd: context lifecycle, parameters, the per-substep stages
+// in the reference's call order (simulation.c:59-108), the fused substep,
+// inspection and event timing.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "sph_internal.h"
+
+namespace sph {
+
+static thread_local std::string g_error;
+
+void set_error(const std::string &msg) { g_error = msg; }
+
+int hip_fail(hipError_t e, const char *what) {
+  g_error = std::string(what) + ": " + hipGetErrorString(e);
+  return e == hipErrorOutOfMemory ? SPH_ERR_NOMEM : SPH_ERR_HIP;
+}
+
+static int arg_error(const char *msg) {
+  g_error = msg;
+  return SPH_ERR_ARG;
+}
+
+static int state_error(const char *msg) {
+  g_error = msg;
+  return SPH_ERR_STATE;
+}
+
+void prof_begin(sph_ctx *c, int kernel, hipEvent_t *a) {
+  (void)kernel;
+  *a = nullptr;
+  if (!c || !c->prof.on) return;
+  hipEvent_t e;
+  if (!c->prof.pool.empty()) {
+    e = c->prof.pool.back();
+    c->prof.pool.pop_back();
+  } else if (hipEventCreate(&e) != hipSuccess) {
+    return;
+  }
+  if (hipEventRecord(e, c->stream) != hipSuccess) {
+    c->prof.pool.push_back(e);
+    return;
+  }
+  *a = e;
+}
+
+void prof_end(sph_ctx *c, int kernel, hipEvent_t a) {
+  if (!c || !c->prof.on || !a) return;
+  hipEvent_t e;
+  if (!c->prof.pool.empty()) {
+    e = c->prof.pool.back();
+    c->prof.pool.pop_back();
+  } else if (hipEventCreate(&e) != hipSuccess) {
+    c->prof.pool.push_back(a);
+    return;
+  }
+  (void)hipEventRecord(e, c->stream);
+  c->prof.pending.push_back({a, e, kernel});
+}
+
+static void prof_collect(sph_ctx *c) {
+  if (c->prof.pending.empty()) return;
+  (void)hipStreamSynchronize(c->stream);
+  for (auto &p : c->prof.pending) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      c->prof.total_ms[p.kernel] += ms;
+      c->prof.launches[p.kernel] += 1;
+    }
+    c->prof.pool.push_back(p.a);
+    c->prof.pool.push_back(p.b);
+  }
+  c->prof.pending.clear();
+}
+
+static const char *kKernelNames[K_COUNT] = {
+    "predict_hash", "hash", "gravity", "predict", "radix_upsweep", "radix_downsweep",
+    "scan", "reorder", "build_lists", "density_lambda", "density", "lambda", "dp",
+    "dp_update", "position_stars", "velocity", "color", "surface", "xsph", "vorticity",
+    "confine", "commit", "aux"};
+
+// Derived FP32 constants (SetParameters, simulation.c:147-177; the constants
+// of particles.c named in DevParams).  Computed in double, rounded once.
+static void derive(sph_ctx *c) {
+  const sph_params &P = c->P;
+  DevParams &d = c->dev;
+  const double h = (double)(float)P.smoothing_radius;
+  const double W_norm = 315.0 / (64.0 * M_PI * pow(h, 9.0));
+  const double DelW_poly_norm = -3.0 * 2.0 * W_norm;
+  const double DelW_norm = -45.0 / (M_PI * pow(h, 6.0));
+  const double C_norm = 32.0 / (M_PI * pow(h, 9.0));
+  const double m = P.rest_mass, rho0 = P.rest_density;
+  d.h = (float)h;
+  d.h2 = (float)(h * h);
+  d.inv_h = (float)(1.0 / h);
+  d.h2_lo = (float)(h * h * (1.0 - 1e-5));
+  d.h2_hi = (float)(h * h * (1.0 + 1e-5));
+  d.h2d = h * h;
+  d.rmin = (float)(h * 0.0001);
+  d.rmin2 = (float)(h * 0.0001 * h * 0.0001);
+  d.W_norm = (float)W_norm;
+  d.DelW_norm = (float)DelW_norm;
+  d.DelW_poly_norm = (float)DelW_poly_norm;
+  d.C_norm = (float)C_norm;
+  d.mass = (float)m;
+  d.rest_density = (float)rho0;
+  d.inv_rest_density = (float)(1.0 / rho0);
+  d.mW0 = (float)(m * W_norm * pow(h, 6.0));
+  d.mWn = (float)(m * W_norm);
+  d.lam_k2 = (float)((m * DelW_norm / rho0) * (m * DelW_norm / rho0));
+  d.lambda_eps = (float)P.lambda_epsilon;
+  d.dt = (float)P.time_step;
+  d.g_dt = (float)(-P.g * P.time_step);
+  d.v_max = (float)P.v_max;
+  d.xsph_c = (float)P.c;
+  d.gamma = (float)P.k;
+  d.coh_k = (float)(-20.0 * P.k * m * m * C_norm);
+  d.curv_k = (float)(-P.k * m);
+  d.h6_64 = (float)(pow(h, 6.0) / 64.0);
+  d.vort_eps = (float)P.vorticity_epsilon;
+  d.vort_model = P.vorticity_model;
+  d.conf_k = (float)(P.vorticity_model == 1 ? P.time_step * P.vorticity_epsilon
+                                            : P.time_step * P.vorticity_epsilon / m);
+  d.color_k = (float)(h * m * DelW_poly_norm);
+  const double lo[3] = {c->box.min_x, c->box.min_y, c->box.min_z};
+  const double hi[3] = {c->box.max_x, c->box.max_y, c->box.max_z};
+  for (int a = 0; a < 3; ++a) {
+    d.box_lo[a] = (float)lo[a];
+    d.box_hi[a] = (float)hi[a];
+    d.box_hi_clamp[a] = (float)(hi[a] - h * 0.0001);
+  }
+}
+
+static int check_params(const sph_params *p) {
+  if (!p) return arg_error("params is NULL");
+  const float hf = (float)p->smoothing_radius;
+  if (!(p->smoothing_radius > 0.0) || (double)hf != p->smoothing_radius)
+    return arg_error("smoothing_radius must be positive and exactly representable in FP32 "
+                     "(the reference uses h = 2*float(spacing))");
+  if (!(p->time_step > 0.0)) return arg_error("time_step must be positive");
+  if (!(p->rest_density > 0.0) || !(p->rest_mass > 0.0))
+    return arg_error("rest_density and rest_mass must be positive");
+  if (p->solve_iterations < 0 || p->solve_iterations > kMaxIters)
+    return arg_error("solve_iterations out of range [0, 32]");
+  if (p->vorticity_model != 0 && p->vorticity_model != 1)
+    return arg_error("vorticity_model must be 0 or 1");
+  return SPH_OK;
+}
+
+static void free_all(sph_ctx *c) {
+  void *bufs[] = {c->x_can, c->v_can, c->xs_can, c->keys, c->keys_alt, c->ids, c->ids_alt,
+                  c->hist, c->cell_count, c->cell_start, c->pos_a, c->pos_b, c->xold,
+                  c->vel_a, c->vel_b, c->aux, c->dpbuf, c->rho, c->list, c->list_cnt,
+                  c->stats, c->red, c->sdf_buf, c->scan.tile_sums};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  for (auto &p : c->prof.pending) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (auto e : c->prof.pool) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+static int require_state(sph_ctx *c) {
+  if (!c) return arg_error("context is NULL");
+  if (!c->state_valid) return state_error("no particle state uploaded");
+  return SPH_OK;
+}
+
+static int require_lists(sph_ctx *c) {
+  int rc = require_state(c);
+  if (rc) return rc;
+  if (!c->lists_valid) return state_error("neighbour lists missing: call sph_find_all_neighbors");
+  return SPH_OK;
+}
+
+}  // namespace sph
+
+using namespace sph;
+
+extern "C" const char *sph_last_error(void) { return g_error.c_str(); }
+
+extern "C" int sph_get_device_count(int *count) {
+  if (!count) return arg_error("count is NULL");
+  SPH_HIP(hipGetDeviceCount(count));
+  return SPH_OK;
+}
+
+extern "C" void sph_default_params(sph_params *p, double h) {
+  if (!p) return;
+  memset(p, 0, sizeof(*p));
+  p->smoothing_radius = h;
+  p->time_step = 1.0 / 300.0;
+  p->g = 9.8;
+  p->k = 0.0;
+  p->c = 0.0;
+  p->rest_density = 1000.0;
+  p->rest_mass = 0.0;
+  p->lambda_epsilon = 0.01;         // particles.c:663
+  p->vorticity_epsilon = 0.1 * h;   // particles.c:289
+  p->v_max = 100.0;                 // particles.c:791
+  p->stiffness_k = 0.95;            // particles.c:693
+  p->density_tolerance = 0.0;
+  p->solve_iterations = 10;         // simulation.c:72
+  p->surface_tension = 1;
+  p->viscosity = 1;
+  p->vorticity = 1;
+  p->vorticity_model = 0;
+}
+
+extern "C" int sph_create(const sph_params *params, const sph_aabb *boundary, int max_particles,
+                          sph_ctx **out) {
+  if (!out) return arg_error("out is NULL");
+  *out = nullptr;
+  int rc = check_params(params);
+  if (rc) return rc;
+  if (!boundary) return arg_error("boundary is NULL");
+  if (boundary->min_x != 0.0 || boundary->min_y != 0.0 || boundary->min_z != 0.0)
+    return arg_error("boundary must start at the origin (the reference hashes absolute "
+                     "coordinates, neighbors.c:86-95)");
+  if (!(boundary->max_x > 0.0 && boundary->max_y > 0.0 && boundary->max_z > 0.0))
+    return arg_error("boundary extent must be positive");
+  if (max_particles < 0) return arg_error("max_particles must be >= 0");
+
+  sph_ctx *c = new sph_ctx();
+  c->P = *params;
+  c->box = *boundary;
+  const double h = params->smoothing_radius;
+  // AllocInitNeighbors, neighbors.c:23-28: ceil(L/h) + 1 cells per axis
+  const double nxd = ceil((boundary->max_x - boundary->min_x) / h) + 1;
+  const double nyd = ceil((boundary->max_y - boundary->min_y) / h) + 1;
+  const double nzd = ceil((boundary->max_z - boundary->min_z) / h) + 1;
+  if (nxd * nyd * nzd >= 4294967295.0) {
+    delete c;
+    return arg_error("hash grid exceeds 2^32 cells");
+  }
+  c->dev.nx = (uint32_t)nxd;
+  c->dev.ny = (uint32_t)nyd;
+  c->dev.nz = (uint32_t)nzd;
+  c->dev.nxy = c->dev.nx * c->dev.ny;
+  c->cells = c->dev.nxy * c->dev.nz;
+  c->key_bits = 1;
+  while (c->key_bits < 32 && ((uint64_t)1 << c->key_bits) < (uint64_t)c->cells) c->key_bits++;
+  derive(c);
+  c->cap = max_particles;
+
+  auto fail = [&](int code) {
+    free_all(c);
+    delete c;
+    return code;
+  };
+  hipError_t e = hipGetDevice(&c->device);
+  if (e != hipSuccess) return fail(hip_fail(e, "hipGetDevice"));
+  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) return fail(hip_fail(e, "hipStreamCreate"));
+
+  const size_t cap = (size_t)std::max(max_particles, 1);
+  const size_t f4 = sizeof(float4) * cap, u4 = sizeof(uint32_t) * cap;
+  const size_t groups = (cap + kWave - 1) / kWave;
+  struct {
+    void **p;
+    size_t bytes;
+  } allocs[] = {
+      {(void **)&c->x_can, f4},       {(void **)&c->v_can, f4},
+      {(void **)&c->xs_can, f4},      {(void **)&c->keys, u4},
+      {(void **)&c->keys_alt, u4},    {(void **)&c->ids, u4},
+      {(void **)&c->ids_alt, u4},     {(void **)&c->cell_count, sizeof(uint32_t) * (c->cells + 1ull)},
+      {(void **)&c->cell_start, sizeof(uint32_t) * (c->cells + 1ull)},
+      {(void **)&c->pos_a, f4},       {(void **)&c->pos_b, f4},
+      {(void **)&c->xold, f4},        {(void **)&c->vel_a, f4},
+      {(void **)&c->vel_b, f4},       {(void **)&c->aux, f4},
+      {(void **)&c->dpbuf, f4},       {(void **)&c->rho, sizeof(float) * cap},
+      {(void **)&c->list, sizeof(uint32_t) * groups * kListStride},
+      {(void **)&c->list_cnt, sizeof(int) * cap},
+      {(void **)&c->stats, sizeof(uint32_t) * STAT_WORDS},
+      {(void **)&c->red, sizeof(double) * 4},
+  };
+  for (auto &a : allocs) {
+    e = hipMalloc(a.p, a.bytes);
+    if (e != hipSuccess) return fail(hip_fail(e, "hipMalloc (context buffers)"));
+  }
+  e = hipMemsetAsync(c->cell_count, 0, sizeof(uint32_t) * (c->cells + 1ull), c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(c->stats, 0, sizeof(uint32_t) * STAT_WORDS, c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(c->list_cnt, 0, sizeof(int) * cap, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return fail(hip_fail(e, "context initialisation"));
+  c->pos = c->pos_a;
+  c->vel = c->vel_a;
+  *out = c;
+  return SPH_OK;
+}
+
+extern "C" int sph_destroy(sph_ctx *c) {
+  if (!c) return SPH_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_all(c);
+  delete c;
+  return SPH_OK;
+}
+
+extern "C" int sph_set_params(sph_ctx *c, const sph_params *p) {
+  if (!c) return arg_error("context is NULL");
+  int rc = check_params(p);
+  if (rc) return rc;
+  if (p->smoothing_radius != c->P.smoothing_radius)
+    return arg_error("smoothing_radius fixes the hash grid; create a new context to change it");
+  c->P = *p;
+  derive(c);
+  return SPH_OK;
+}
+
+extern "C" int sph_get_params(sph_ctx *c, sph_params *p) {
+  if (!c || !p) return arg_error("NULL argument");
+  *p = c->P;
+  return SPH_OK;
+}
+
+extern "C" int sph_grid_dims(sph_ctx *c, int *nx, int *ny, int *nz) {
+  if (!c) return arg_error("context is NULL");
+  if (nx) *nx = (int)c->dev.nx;
+  if (ny) *ny = (int)c->dev.ny;
+  if (nz) *nz = (int)c->dev.nz;
+  return SPH_OK;
+}
+
+static int make_sdf(const sph_obstacle_desc *o, SdfDev *d, float **buf, hipStream_t s) {
+  if (o->dim_x < 1 || o->dim_y < 1 || o->dim_z < 1 || !(o->spacing > 0.0))
+    return arg_error("obstacle dims and spacing must be positive");
+  const sph_aabb &b = o->world_bounds;
+  if (!(b.max_x > b.min_x && b.max_y > b.min_y && b.max_z > b.min_z))
+    return arg_error("obstacle world bounds must be non-empty");
+  const size_t count = (size_t)o->dim_x * (size_t)o->dim_y * (size_t)o->dim_z;
+  SPH_HIP(hipMalloc(buf, sizeof(float) * count));
+  SPH_HIP(hipMemcpyAsync(*buf, o->sdf, sizeof(float) * count, hipMemcpyHostToDevice, s));
+  SPH_HIP(hipStreamSynchronize(s));
+  d->data = *buf;
+  d->dim[0] = o->dim_x;
+  d->dim[1] = o->dim_y;
+  d->dim[2] = o->dim_z;
+  const double lo[3] = {b.min_x, b.min_y, b.min_z}, hi[3] = {b.max_x, b.max_y, b.max_z};
+  for (int a = 0; a < 3; ++a) {
+    d->wmin[a] = (float)lo[a];
+    d->wmax[a] = (float)hi[a];
+    d->len[a] = (float)(hi[a] - lo[a]);
+    d->inv_dim[a] = (float)(1.0 / (double)d->dim[a]);
+  }
+  const float len_model = (float)(o->dim_x * o->spacing);  // obstacle_cuda.cu:101
+  d->m2w = d->len[0] / len_model;                          // obstacle_cuda.cu:105
+  return SPH_OK;
+}
+
+extern "C" int sph_set_obstacle(sph_ctx *c, const sph_obstacle_desc *o) {
+  if (!c) return arg_error("context is NULL");
+  if (c->sdf_buf) {
+    SPH_HIP(hipStreamSynchronize(c->stream));
+    SPH_HIP(hipFree(c->sdf_buf));
+    c->sdf_buf = nullptr;
+  }
+  memset(&c->sdf, 0, sizeof(c->sdf));
+  if (!o || !o->sdf) return SPH_OK;
+  int rc = make_sdf(o, &c->sdf, &c->sdf_buf, c->stream);
+  if (rc) {
+    memset(&c->sdf, 0, sizeof(c->sdf));
+    if (c->sdf_buf) (void)hipFree(c->sdf_buf);
+    c->sdf_buf = nullptr;
+  }
+  return rc;
+}
+
+extern "C" int sph_upload_state(sph_ctx *c, int n, const float *x, const float *y,
+                                const float *z, const float *vx, const float *vy,
+                                const float *vz) {
+  if (!c) return arg_error("context is NULL");
+  if (n < 0) return arg_error("n must be >= 0");
+  if (n > c->cap) {
+    set_error("particle count exceeds the context capacity");
+    return SPH_ERR_CAPACITY;
+  }
+  if (n > 0 && (!x || !y || !z || !vx || !vy || !vz)) return arg_error("NULL state array");
+  std::vector<float4> hx((size_t)n), hv((size_t)n);
+  for (int i = 0; i < n; ++i) {
+    hx[i] = make_float4(x[i], y[i], z[i], 0.0f);
+    hv[i] = make_float4(vx[i], vy[i], vz[i], 0.0f);
+  }
+  SPH_HIP(hipStreamSynchronize(c->stream));
+  if (n) {
+    SPH_HIP(hipMemcpyAsync(c->x_can, hx.data(), sizeof(float4) * n, hipMemcpyHostToDevice,
+                           c->stream));
+    SPH_HIP(hipMemcpyAsync(c->v_can, hv.data(), sizeof(float4) * n, hipMemcpyHostToDevice,
+                           c->stream));
+  }
+  SPH_HIP(hipStreamSynchronize(c->stream));
+  c->n = n;
+  c->state_valid = true;
+  c->sorted_valid = false;
+  c->lists_valid = false;
+  return SPH_OK;
+}
+
+extern "C" int sph_download_state(sph_ctx *c, float *x, float *y, float *z, float *vx,
+                                  float *vy, float *vz) {
+  int rc = require_state(c);
+  if (rc) return rc;
+  const int n = c->n;
+  std::vector<float4> hx((size_t)n), hv((size_t)n);
+  if (n) {
+    SPH_HIP(hipMemcpyAsync(hx.data(), c->x_can, sizeof(float4) * n, hipMemcpyDeviceToHost,
+                           c->stream));
+    SPH_HIP(hipMemcpyAsync(hv.data(), c->v_can, sizeof(float4) * n, hipMemcpyDeviceToHost,
+                           c->stream));
+  }
+  SPH_HIP(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < n; ++i) {
+    if (x) x[i] = hx[i].x;
+    if (y) y[i] = hx[i].y;
+    if (z) z[i] = hx[i].z;
+    if (vx) vx[i] = hv[i].x;
+    if (vy) vy[i] = hv[i].y;
+    if (vz) vz[i] = hv[i].z;
+  }
+  return SPH_OK;
+}
+
+extern "C" int sph_particle_count(sph_ctx *c, int *n) {
+  if (!c || !n) return arg_error("NULL argument");
+  *n = c->n;
+  return SPH_OK;
+}
+
+// --------------------------------------------------------------------------
+// Stages in the driver's order
+// --------------------------------------------------------------------------
+extern "C" int sph_apply_gravity(sph_ctx *c) {
+  int rc = require_state(c);
+  return rc ? rc : launch_gravity(c);
+}
+
+extern "C" int sph_predict_positions(sph_ctx *c) {
+  int rc = require_state(c);
+  return rc ? rc : launch_predict(c);
+}
+
+extern "C" int sph_find_all_neighbors(sph_ctx *c) {
+  int rc = require_state(c);
+  if (rc) return rc;
+  rc = launch_substep_init(c);
+  if (rc) return rc;
+  c->staged_slot = 0;
+  return launch_neighbor_search(c, false);
+}
+
+extern "C" int sph_compute_densities(sph_ctx *c) {
+  int rc = require_lists(c);
+  if (rc) return rc;
+  const int slot = std::min(c->staged_slot, kMaxIters);
+  c->staged_slot++;
+  c->last_err_slot = slot;
+  // density_to_w keeps rho in pos.w for the velocity passes
+  return launch_density(c, 3, slot);
+}
+
+extern "C" int sph_compute_lambda(sph_ctx *c) {
+  int rc = require_lists(c);
+  return rc ? rc : launch_density(c, 2, -1);
+}
+
+extern "C" int sph_update_dps(sph_ctx *c, int substep) {
+  int rc = require_lists(c);
+  if (rc) return rc;
+  if (substep < 0) return arg_error("substep must be >= 0");
+  return launch_dp(c, substep);
+}
+
+extern "C" int sph_update_position_stars(sph_ctx *c) {
+  int rc = require_lists(c);
+  return rc ? rc : launch_position_stars(c);
+}
+
+extern "C" int sph_update_velocities(sph_ctx *c) {
+  int rc = require_lists(c);
+  return rc ? rc : launch_velocity(c);
+}
+
+extern "C" int sph_compute_surface_tension(sph_ctx *c) {
+  int rc = require_lists(c);
+  if (rc) return rc;
+  rc = launch_color(c, false);
+  return rc ? rc : launch_surface(c);
+}
+
+extern "C" int sph_apply_viscosity(sph_ctx *c) {
+  int rc = require_lists(c);
+  return rc ? rc : launch_xsph(c);
+}
+
+extern "C" int sph_compute_vorticity(sph_ctx *c) {
+  int rc = require_lists(c);
+  return rc ? rc : launch_vorticity(c);
+}
+
+extern "C" int sph_apply_vorticity_confinement(sph_ctx *c) {
+  int rc = require_lists(c);
+  return rc ? rc : launch_confine(c, false);
+}
+
+extern "C" int sph_update_positions(sph_ctx *c) {
+  int rc = require_lists(c);
+  if (rc) return rc;
+  rc = launch_commit(c);
+  if (rc) return rc;
+  c->substeps++;
+  return SPH_OK;
+}
+
+// --------------------------------------------------------------------------
+// Fused substep
+// --------------------------------------------------------------------------
+static int substep_fused(sph_ctx *c) {
+  int rc = launch_substep_init(c);
+  if (rc) return rc;
+  rc = launch_neighbor_search(c, true);
+  if (rc) return rc;
+  const int iters = c->P.solve_iterations;
+  for (int it = 0; it < iters; ++it) {
+    if ((rc = launch_density(c, 0, it))) return rc;
+    if ((rc = launch_dp_update(c, it, it))) return rc;
+  }
+  c->last_err_slot = iters > 0 ? iters - 1 : -1;
+  if (c->P.density_tolerance > 0.0) {
+    // density after the last allowed update; skipped when already converged
+    if ((rc = launch_density(c, 3, iters))) return rc;
+    c->last_err_slot = iters;
+  }
+  if (c->P.surface_tension) {
+    if ((rc = launch_color(c, true))) return rc;
+    if ((rc = launch_surface(c))) return rc;
+  } else {
+    if ((rc = launch_velocity(c))) return rc;
+  }
+  if (c->P.viscosity && (rc = launch_xsph(c))) return rc;
+  if (c->P.vorticity) {
+    if ((rc = launch_vorticity(c))) return rc;
+    if ((rc = launch_confine(c, true))) return rc;
+  } else {
+    if ((rc = launch_commit(c))) return rc;
+  }
+  c->substeps++;
+  return SPH_OK;
+}
+
+extern "C" int sph_step(sph_ctx *c, int substeps) {
+  int rc = require_state(c);
+  if (rc) return rc;
+  if (substeps < 0) return arg_error("substeps must be >= 0");
+  for (int s = 0; s < substeps; ++s) {
+    rc = substep_fused(c);
+    if (rc) return rc;
+  }
+  return SPH_OK;
+}
+
+extern "C" int sph_synchronize(sph_ctx *c) {
+  if (!c) return arg_error("context is NULL");
+  SPH_HIP(hipStreamSynchronize(c->stream));
+  return SPH_OK;
+}
+
+extern "C" int sph_get_stats(sph_ctx *c, sph_stats *st) {
+  if (!c || !st) return arg_error("NULL argument");
+  uint32_t w[STAT_WORDS];
+  SPH_HIP(hipMemcpyAsync(w, c->stats, sizeof(w), hipMemcpyDeviceToHost, c->stream));
+  SPH_HIP(hipStreamSynchronize(c->stream));
+  memset(st, 0, sizeof(*st));
+  st->particles = c->n;
+  st->iterations = (int)w[STAT_ITERS];
+  st->converged = (int)w[STAT_CONVERGED];
+  int slot = c->last_err_slot;
+  if (c->P.density_tolerance > 0.0 && st->converged) slot = st->iterations;
+  if (slot >= 0 && slot <= kMaxIters) {
+    float f;
+    memcpy(&f, &w[STAT_ERR0 + slot], sizeof(f));
+    st->max_density_error = f;
+  }
+  st->neighbor_overflow = (unsigned long long)w[STAT_DROPPED] |
+                          ((unsigned long long)w[STAT_DROPPED + 1] << 32);
+  st->cell_clamps =
+      (unsigned long long)w[STAT_CLAMPS] | ((unsigned long long)w[STAT_CLAMPS + 1] << 32);
+  st->substeps = c->substeps;
+  return SPH_OK;
+}
+
+extern "C" int sph_compute_energy(sph_ctx *c, double *ke) {
+  int rc = require_state(c);
+  if (rc) return rc;
+  if (!ke) return arg_error("NULL argument");
+  rc = launch_kinetic_energy(c, c->red);
+  if (rc) return rc;
+  SPH_HIP(hipMemcpyAsync(ke, c->red, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  SPH_HIP(hipStreamSynchronize(c->stream));
+  return SPH_OK;
+}
+
+// --------------------------------------------------------------------------
+// Inspection
+// --------------------------------------------------------------------------
+template <class T>
+static int fetch(sph_ctx *c, const T *dev, size_t count, std::vector<T> *out) {
+  out->resize(count);
+  if (count)
+    SPH_HIP(hipMemcpyAsync(out->data(), dev, sizeof(T) * count, hipMemcpyDeviceToHost,
+                           c->stream));
+  SPH_HIP(hipStreamSynchronize(c->stream));
+  return SPH_OK;
+}
+
+extern "C" int sph_download_field(sph_ctx *c, int field, float *out) {
+  int rc = require_lists(c);
+  if (rc) return rc;
+  if (!out) return arg_error("out is NULL");
+  const int n = c->n;
+  std::vector<uint32_t> ids;
+  if ((rc = fetch(c, c->id_s, (size_t)n, &ids))) return rc;
+  std::vector<float4> v4;
+  std::vector<float> v1;
+  int comps = 3;
+  switch (field) {
+    case SPH_FIELD_POSITION_STAR: rc = fetch(c, (const float4 *)c->pos, n, &v4); break;
+    case SPH_FIELD_DP: rc = fetch(c, (const float4 *)c->dpbuf, n, &v4); break;
+    case SPH_FIELD_VELOCITY: rc = fetch(c, (const float4 *)c->vel, n, &v4); break;
+    case SPH_FIELD_COLOR:
+    case SPH_FIELD_VORTICITY: rc = fetch(c, (const float4 *)c->aux, n, &v4); break;
+    case SPH_FIELD_POSITION_OLD: rc = fetch(c, (const float4 *)c->xold, n, &v4); break;
+    case SPH_FIELD_DENSITY: comps = 1; rc = fetch(c, (const float *)c->rho, n, &v1); break;
+    case SPH_FIELD_LAMBDA:
+      comps = 1;
+      rc = fetch(c, (const float4 *)c->pos, n, &v4);
+      break;
+    default: return arg_error("unknown field");
+  }
+  if (rc) return rc;
+  for (int s = 0; s < n; ++s) {
+    const uint32_t id = ids[s];
+    if (field == SPH_FIELD_DENSITY) {
+      out[id] = v1[s];
+    } else if (field == SPH_FIELD_LAMBDA) {
+      out[id] = v4[s].w;
+    } else {
+      out[3 * (size_t)id + 0] = v4[s].x;
+      out[3 * (size_t)id + 1] = v4[s].y;
+      out[3 * (size_t)id + 2] = v4[s].z;
+    }
+  }
+  (void)comps;
+  return SPH_OK;
+}
+
+extern "C" int sph_download_neighbors(sph_ctx *c, uint32_t *out_ids, int *counts) {
+  int rc = require_lists(c);
+  if (rc) return rc;
+  if (!out_ids || !counts) return arg_error("NULL argument");
+  const int n = c->n;
+  const size_t groups = ((size_t)n + kWave - 1) / kWave;
+  std::vector<uint32_t> ids, list;
+  std::vector<int> cnt;
+  if ((rc = fetch(c, c->id_s, (size_t)n, &ids))) return rc;
+  if ((rc = fetch(c, (const int *)c->list_cnt, (size_t)n, &cnt))) return rc;
+  if ((rc = fetch(c, (const uint32_t *)c->list, groups * kListStride, &list))) return rc;
+  for (int s = 0; s < n; ++s) {
+    const uint32_t id = ids[s];
+    counts[id] = cnt[s];
+    const size_t base = (size_t)(s >> 6) * kListStride + (s & 63);
+    for (int k = 0; k < cnt[s]; ++k)
+      out_ids[(size_t)id * kMaxNb + k] = ids[list[base + (size_t)k * kWave]];
+  }
+  return SPH_OK;
+}
+
+extern "C" int sph_download_sorted(sph_ctx *c, uint32_t *keys, uint32_t *ids) {
+  int rc = require_lists(c);
+  if (rc) return rc;
+  const int n = c->n;
+  if (n && keys) SPH_HIP(hipMemcpyAsync(keys, c->key_s, 4ull * n, hipMemcpyDeviceToHost, c->stream));
+  if (n && ids) SPH_HIP(hipMemcpyAsync(ids, c->id_s, 4ull * n, hipMemcpyDeviceToHost, c->stream));
+  SPH_HIP(hipStreamSynchronize(c->stream));
+  return SPH_OK;
+}
+
+extern "C" int sph_download_cell_bounds(sph_ctx *c, uint32_t *start, uint32_t *end) {
+  int rc = require_lists(c);
+  if (rc) return rc;
+  std::vector<uint32_t> cs;
+  if ((rc = fetch(c, (const uint32_t *)c->cell_start, (size_t)c->cells + 1, &cs))) return rc;
+  for (uint32_t i = 0; i < c->cells; ++i) {
+    if (start) start[i] = cs[i];
+    if (end) end[i] = cs[i + 1];
+  }
+  return SPH_OK;
+}
+
+// --------------------------------------------------------------------------
+// Standalone collision and device plumbing
+// --------------------------------------------------------------------------
+struct sph_sdf {
+  SdfDev dev;
+  float *buf;
+};
+
+extern "C" int sph_sdf_create(const sph_obstacle_desc *desc, sph_sdf **out) {
+  if (!desc || !desc->sdf || !out) return arg_error("NULL argument");
+  *out = nullptr;
+  sph_sdf *s = new sph_sdf();
+  s->buf = nullptr;
+  int rc = make_sdf(desc, &s->dev, &s->buf, nullptr);
+  if (rc) {
+    if (s->buf) (void)hipFree(s->buf);
+    delete s;
+    return rc;
+  }
+  *out = s;
+  return SPH_OK;
+}
+
+extern "C" int sph_sdf_destroy(sph_sdf *s) {
+  if (!s) return SPH_OK;
+  if (s->buf) (void)hipFree(s->buf);
+  delete s;
+  return SPH_OK;
+}
+
+extern "C" int sph_calculate_particle_collisions(float *x, float *y, float *z, int n,
+                                                 const sph_sdf *sdf, void *stream) {
+  if (n < 0 || !sdf || (n > 0 && (!x || !y || !z))) return arg_error("invalid argument");
+  int rc = launch_collide_soa(x, y, z, (uint32_t)n, sdf->dev, (hipStream_t)stream);
+  if (rc) return rc;
+  SPH_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return SPH_OK;
+}
+
+extern "C" int sph_device_malloc(void **ptr, size_t bytes) {
+  if (!ptr) return arg_error("ptr is NULL");
+  SPH_HIP(hipMalloc(ptr, bytes ? bytes : 1));
+  return SPH_OK;
+}
+
+extern "C" int sph_device_free(void *ptr) {
+  if (ptr) SPH_HIP(hipFree(ptr));
+  return SPH_OK;
+}
+
+extern "C" int sph_memcpy_htod(void *dst, const void *src, size_t bytes) {
+  if (bytes) SPH_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return SPH_OK;
+}
+
+extern "C" int sph_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
+  if (bytes) SPH_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return SPH_OK;
+}
+
+// --------------------------------------------------------------------------
+// Timing
+// --------------------------------------------------------------------------
+extern "C" int sph_profile_enable(sph_ctx *c, int enable) {
+  if (!c) return arg_error("context is NULL");
+  if (!enable) prof_collect(c);
+  c->prof.on = enable != 0;
+  return SPH_OK;
+}
+
+extern "C" int sph_profile_read(sph_ctx *c, int kernel, double *total_ms, long long *launches) {
+  if (!c) return arg_error("context is NULL");
+  if (kernel < 0 || kernel >= K_COUNT) return arg_error("unknown kernel id");
+  prof_collect(c);
+  if (total_ms) *total_ms = c->prof.total_ms[kernel];
+  if (launches) *launches = c->prof.launches[kernel];
+  return SPH_OK;
+}
+
+extern "C" int sph_profile_reset(sph_ctx *c) {
+  if (!c) return arg_error("context is NULL");
+  prof_collect(c);
+  for (int k = 0; k < K_COUNT; ++k) {
+    c->prof.total_ms[k] = 0.0;
+    c->prof.launches[k] = 0;
+  }
+  return SPH_OK;
+}
+
+extern "C" const char *sph_profile_kernel_name(int kernel) {
+  return (kernel >= 0 && kernel < K_COUNT) ? kKernelNames[kernel] : "";
+}
+
+extern "C" int sph_profile_kernel_count(void) { return K_COUNT; }

@@ -1,0 +1,178 @@
+// Internal declarations shared by the HIP translation units of libsphpc_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "sphpc_amd.h"
+
+namespace sph {
+
+constexpr int kMaxNb = SPH_MAX_NEIGHBORS;
+constexpr int kBlock = 256;          // threads per block for particle kernels
+constexpr int kWave = 64;            // CDNA wavefront
+constexpr int kListStride = kWave * kMaxNb;  // list words per 64-particle group
+
+// Device-side constants of one run, all FP32 except the exact-predicate h^2.
+struct DevParams {
+  float h, h2, inv_h;
+  float h2_lo, h2_hi;        // guard band around h^2 for the list predicate
+  double h2d;                // (double)h * (double)h, exact
+  float rmin, rmin2;         // r floor 1e-4 h (particles.c:125,265,344,636,725)
+  float W_norm, DelW_norm, DelW_poly_norm, C_norm;
+  float mass, rest_density, inv_rest_density;
+  float mW0;                 // m * W(0)
+  float mWn;                 // m * W_norm
+  float lam_k2;              // (m * DelW_norm / rho0)^2
+  float lambda_eps;
+  float dt, g_dt;            // g_dt = -g * dt
+  float v_max;
+  float xsph_c, gamma;
+  float coh_k;               // -20 * gamma * m^2 * C_norm
+  float curv_k;              // -gamma * m
+  float h6_64;               // h^6 / 64
+  float vort_eps;
+  float conf_k;              // dt * vort_eps / m  (model 0) or dt * vort_eps (model 1)
+  float color_k;             // h * m * DelW_poly_norm
+  float box_lo[3], box_hi[3], box_hi_clamp[3];  // clamp to [lo, hi - 1e-4 h)
+  uint32_t nx, ny, nz, nxy;
+  int vort_model;
+};
+
+// Obstacle grid in device memory, sampled with software trilinear filtering.
+struct SdfDev {
+  const float *data;  // nullptr: no obstacle
+  int dim[3];
+  float wmin[3], wmax[3];
+  float len[3];       // float(wmax - wmin)
+  float inv_dim[3];   // 1/dim (texel size in normalised coordinates)
+  float m2w;          // world length / model length (obstacle_cuda.cu:101-105)
+};
+
+enum StatSlot {
+  STAT_DROPPED = 0,      // 64-bit counter (two words)
+  STAT_CLAMPS = 2,       // 64-bit counter (two words)
+  STAT_ITERS = 4,
+  STAT_CONVERGED = 5,
+  STAT_ERR0 = 8,         // per-iteration max error bits (kMaxIters + 1 slots)
+  STAT_WORDS = 64
+};
+constexpr int kMaxIters = 32;
+
+enum KernelId {
+  K_PREDICT_HASH = 0, K_HASH, K_GRAVITY, K_PREDICT, K_RADIX_UP, K_RADIX_DOWN, K_SCAN,
+  K_REORDER, K_BUILD_LISTS, K_DENSITY_LAMBDA, K_DENSITY, K_LAMBDA, K_DP, K_DP_UPDATE,
+  K_POS_STARS, K_VELOCITY, K_COLOR, K_SURFACE, K_XSPH, K_VORTICITY, K_CONFINE, K_COMMIT,
+  K_AUX, K_COUNT
+};
+
+struct ScanScratch {
+  uint32_t *tile_sums = nullptr;
+  size_t tile_cap = 0;
+};
+
+struct Profiler {
+  bool on = false;
+  struct Pair { hipEvent_t a, b; int kernel; };
+  std::vector<Pair> pending;
+  std::vector<hipEvent_t> pool;
+  double total_ms[K_COUNT] = {};
+  long long launches[K_COUNT] = {};
+};
+
+}  // namespace sph
+
+struct sph_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  sph_params P{};
+  sph_aabb box{};
+  sph::DevParams dev{};
+  sph::SdfDev sdf{};
+  float *sdf_buf = nullptr;
+
+  uint32_t cells = 0;
+  int key_bits = 1;
+  int cap = 0;
+  int n = 0;
+
+  // canonical (particle-id order) state
+  float4 *x_can = nullptr, *v_can = nullptr, *xs_can = nullptr;
+  // sort
+  uint32_t *keys = nullptr, *keys_alt = nullptr, *ids = nullptr, *ids_alt = nullptr;
+  uint32_t *hist = nullptr;
+  size_t hist_cap = 0;
+  const uint32_t *key_s = nullptr, *id_s = nullptr;  // final sorted buffers
+  uint32_t *cell_count = nullptr, *cell_start = nullptr;  // cells + 1 entries
+  sph::ScanScratch scan;
+  // sorted working set
+  float4 *pos_a = nullptr, *pos_b = nullptr, *pos = nullptr;  // pos: current x*
+  float4 *xold = nullptr;
+  float4 *vel_a = nullptr, *vel_b = nullptr, *vel = nullptr;  // vel: current v
+  float4 *aux = nullptr;  // colour normal, then vorticity (w = |omega|)
+  float4 *dpbuf = nullptr;  // staged delta p
+  float *rho = nullptr;
+  uint32_t *list = nullptr;
+  int *list_cnt = nullptr;
+  uint32_t *stats = nullptr;  // sph::STAT_WORDS words
+  double *red = nullptr;      // FP64 reduction scratch
+  int last_err_slot = -1;
+  int staged_slot = 0;
+  bool state_valid = false;
+  bool sorted_valid = false;
+  bool lists_valid = false;
+  long long substeps = 0;
+  sph::Profiler prof;
+};
+
+namespace sph {
+
+void set_error(const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+
+#define SPH_HIP(call)                                   \
+  do {                                                  \
+    hipError_t e_ = (call);                             \
+    if (e_ != hipSuccess) return sph::hip_fail(e_, #call); \
+  } while (0)
+
+inline uint32_t blocks_for(uint32_t n, uint32_t per = kBlock) { return (n + per - 1) / per; }
+
+// Launch helpers with optional event timing.
+void prof_begin(sph_ctx *c, int kernel, hipEvent_t *a);
+void prof_end(sph_ctx *c, int kernel, hipEvent_t a);
+
+// Sort / scan (sph_sort.hip).
+int exclusive_scan(uint32_t *in, uint32_t *out, uint32_t len, bool zero_in, bool write_total,
+                   ScanScratch *scratch, hipStream_t s, sph_ctx *prof);
+int radix_sort_pairs(const uint32_t *kin, const uint32_t *vin, uint32_t *k0, uint32_t *v0,
+                     uint32_t *k1, uint32_t *v1, uint32_t n, int key_bits, uint32_t **hist,
+                     size_t *hist_cap, ScanScratch *scratch, hipStream_t s, sph_ctx *prof,
+                     const uint32_t **kout, const uint32_t **vout);
+int count_keys(const uint32_t *keys, uint32_t n, uint32_t *count, uint32_t cells,
+               hipStream_t s);
+
+// Physics launchers (sph_physics.hip).
+int launch_gravity(sph_ctx *c);
+int launch_predict(sph_ctx *c);
+int launch_neighbor_search(sph_ctx *c, bool fused);
+int launch_density(sph_ctx *c, int mode, int err_slot);
+int launch_dp(sph_ctx *c, int substep);
+int launch_position_stars(sph_ctx *c);
+int launch_dp_update(sph_ctx *c, int substep, int err_slot);
+int launch_velocity(sph_ctx *c);
+int launch_color(sph_ctx *c, bool with_velocity);
+int launch_surface(sph_ctx *c);
+int launch_xsph(sph_ctx *c);
+int launch_vorticity(sph_ctx *c);
+int launch_confine(sph_ctx *c, bool commit);
+int launch_commit(sph_ctx *c);
+int launch_substep_init(sph_ctx *c);
+int launch_collide_soa(float *x, float *y, float *z, uint32_t n, const SdfDev &o,
+                       hipStream_t s);
+int launch_kinetic_energy(sph_ctx *c, double *out_dev);
+
+}  // namespace sph

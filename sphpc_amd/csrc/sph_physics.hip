@@ -1,0 +1,983 @@
+// Position-based-fluids substep kernels for gfx950.
+//
+// Particles live in two orders.  The canonical state (x_can, v_can) is in
+// particle-id order, like the reference's arrays.  After the neighbour search
+// the substep works on copies reordered into (cell key, id) order, so that a
+// particle's 27-cell neighbourhood is nine contiguous index ranges and the
+// neighbour gathers of one wavefront hit the same cache lines.  The commit
+// kernel scatters x* and v back to id order at the end of the substep.
+//
+// Vector fields are float4.  pos.w carries lambda between the density/lambda
+// pass and the delta-p pass, and rho afterwards; aux.w carries |omega|.
+//
+// Neighbour lists are stored per 64-particle group, slot-major
+// (list[(s/64)*64*60 + k*64 + s%64]), so lane s of a wave reading its k-th
+// neighbour index reads 64 consecutive words.
+#include "sph_internal.h"
+
+namespace sph {
+
+// Logical block index for a launch of nb blocks so that each XCD (blocks are
+// dispatched round-robin over 8 XCDs) walks one contiguous range of particles
+// and keeps its neighbourhood in its own L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+  const uint32_t q = nb >> 3, r = nb & 7u;
+  const uint32_t x = b & 7u, idx = b >> 3;
+  return x * q + min(x, r) + idx;
+}
+
+__device__ __forceinline__ uint32_t particle_index(uint32_t n_blocks) {
+  return xcd_block(blockIdx.x, n_blocks) * kBlock + threadIdx.x;
+}
+
+__device__ __forceinline__ const uint32_t *list_ptr(const uint32_t *list, uint32_t s) {
+  return list + (size_t)(s >> 6) * kListStride + (s & 63u);
+}
+
+__device__ __forceinline__ float fast_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// ApplyBoundaryConditions (particles.c:842-867).
+__device__ __forceinline__ float clamp_axis(float x, float lo, float hi, float hi_clamp) {
+  if (x < lo) return lo;
+  if (x >= hi) return hi_clamp;
+  return x;
+}
+
+__device__ __forceinline__ void apply_boundary(float &x, float &y, float &z,
+                                               const DevParams &P) {
+  x = clamp_axis(x, P.box_lo[0], P.box_hi[0], P.box_hi_clamp[0]);
+  y = clamp_axis(y, P.box_lo[1], P.box_hi[1], P.box_hi_clamp[1]);
+  z = clamp_axis(z, P.box_lo[2], P.box_hi[2], P.box_hi_clamp[2]);
+}
+
+// Exact floor(x / h) for float x, h: a float estimate corrected with the sign
+// of the exactly rounded residual fma(-k, h, x).  Equals the reference's
+// floor of the double quotient (neighbors.c:86-88), which never rounds across
+// an integer for float operands.
+__device__ __forceinline__ uint32_t cell_coord(float x, float h, float inv_h, uint32_t dim,
+                                               uint32_t *clamped) {
+  float k = floorf(x * inv_h);
+  if (fmaf(-k, h, x) < 0.0f)
+    k -= 1.0f;
+  else if (fmaf(-(k + 1.0f), h, x) >= 0.0f)
+    k += 1.0f;
+  if (!(k >= 0.0f)) {  // negative or NaN: keep the index inside the grid
+    *clamped += 1u;
+    return 0u;
+  }
+  if (k > (float)(dim - 1)) {
+    *clamped += 1u;
+    return dim - 1;
+  }
+  return (uint32_t)k;
+}
+
+// HashVal (neighbors.c:79-98).
+__device__ __forceinline__ uint32_t hash_key(float x, float y, float z, const DevParams &P,
+                                             uint32_t *clamped) {
+  const uint32_t gx = cell_coord(x, P.h, P.inv_h, P.nx, clamped);
+  const uint32_t gy = cell_coord(y, P.h, P.inv_h, P.ny, clamped);
+  const uint32_t gz = cell_coord(z, P.h, P.inv_h, P.nz, clamped);
+  return P.nxy * gz + (gy * P.nx + gx);
+}
+
+__device__ __forceinline__ void add_counter64(uint32_t *stats, int slot, uint32_t v) {
+  if (v) atomicAdd(reinterpret_cast<unsigned long long *>(stats + slot), (unsigned long long)v);
+}
+
+// ---------------------------------------------------------------------------
+// Elementwise stages before the neighbour search
+// ---------------------------------------------------------------------------
+// ApplyGravity (particles.c:516-527).
+__global__ __launch_bounds__(kBlock) void k_gravity(uint32_t n, float4 *__restrict__ v,
+                                                    DevParams P) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  v[i].z = v[i].z + P.g_dt;
+}
+
+// PredictPositions (particles.c:748-784).
+__global__ __launch_bounds__(kBlock) void k_predict(uint32_t n, const float4 *__restrict__ x,
+                                                    const float4 *__restrict__ v,
+                                                    float4 *__restrict__ xs, DevParams P) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float4 a = x[i], u = v[i];
+  float px = a.x + u.x * P.dt, py = a.y + u.y * P.dt, pz = a.z + u.z * P.dt;
+  apply_boundary(px, py, pz, P);
+  xs[i] = make_float4(px, py, pz, 0.0f);
+}
+
+// HashParticles (neighbors.c:101-125) plus the cell histogram used for the
+// cell tables.  count must be zero on entry (the scan re-zeroes it).
+__global__ __launch_bounds__(kBlock) void k_hash(uint32_t n, const float4 *__restrict__ xs,
+                                                 uint32_t *__restrict__ keys,
+                                                 uint32_t *__restrict__ count, DevParams P,
+                                                 uint32_t *__restrict__ stats) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float4 a = xs[i];
+  uint32_t clamped = 0;
+  const uint32_t key = hash_key(a.x, a.y, a.z, P, &clamped);
+  keys[i] = key;
+  atomicAdd(&count[key], 1u);
+  add_counter64(stats, STAT_CLAMPS, clamped);
+}
+
+// Fused ApplyGravity + PredictPositions + HashParticles for the fused
+// substep: v after gravity is only read by the prediction (UpdateVelocities
+// overwrites it), so it is not stored.
+__global__ __launch_bounds__(kBlock) void k_predict_hash(uint32_t n,
+                                                         const float4 *__restrict__ x,
+                                                         const float4 *__restrict__ v,
+                                                         float4 *__restrict__ xs,
+                                                         uint32_t *__restrict__ keys,
+                                                         uint32_t *__restrict__ count,
+                                                         DevParams P,
+                                                         uint32_t *__restrict__ stats) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float4 a = x[i], u = v[i];
+  const float vz = u.z + P.g_dt;
+  float px = a.x + u.x * P.dt, py = a.y + u.y * P.dt, pz = a.z + vz * P.dt;
+  apply_boundary(px, py, pz, P);
+  xs[i] = make_float4(px, py, pz, 0.0f);
+  uint32_t clamped = 0;
+  const uint32_t key = hash_key(px, py, pz, P, &clamped);
+  keys[i] = key;
+  atomicAdd(&count[key], 1u);
+  add_counter64(stats, STAT_CLAMPS, clamped);
+}
+
+// Gather x* and x into (cell key, id) order.
+__global__ __launch_bounds__(kBlock) void k_reorder(uint32_t n, const uint32_t *__restrict__ id_s,
+                                                    const float4 *__restrict__ xs_can,
+                                                    const float4 *__restrict__ x_can,
+                                                    float4 *__restrict__ pos,
+                                                    float4 *__restrict__ xold) {
+  const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= n) return;
+  const uint32_t id = id_s[s];
+  pos[s] = xs_can[id];
+  xold[s] = x_can[id];
+}
+
+// ---------------------------------------------------------------------------
+// Neighbour lists: FillNeighbors (neighbors.c:217-298)
+// ---------------------------------------------------------------------------
+// r^2 < h^2 evaluated exactly as the reference does it, in double with the
+// reference's association and no contraction (neighbors.c:278-285).
+__device__ __noinline__ bool inside_exact(float px, float py, float pz, float qx, float qy,
+                                          float qz, double h2d) {
+#pragma clang fp contract(off)
+  const double dx = (double)px - (double)qx;
+  const double dy = (double)py - (double)qy;
+  const double dz = (double)pz - (double)qz;
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  return r2 < h2d;
+}
+
+// One thread per sorted particle.  The 3 cells (dx = -1..1) of each (dz, dy)
+// row are consecutive keys, so each row is one contiguous range of sorted
+// indices; walking rows in (dz, dy) order and indices upward reproduces the
+// reference's (dz, dy, dx, id) order and therefore its 60-cap truncation.
+__global__ __launch_bounds__(kBlock) void k_build_lists(
+    uint32_t n, uint32_t nblocks, const float4 *__restrict__ pos,
+    const uint32_t *__restrict__ key_s, const uint32_t *__restrict__ cstart, DevParams P,
+    uint32_t *__restrict__ list, int *__restrict__ cnt, uint32_t *__restrict__ stats) {
+  const uint32_t s = particle_index(nblocks);
+  if (s >= n) return;
+  const float4 p = pos[s];
+  const uint32_t key = key_s[s];
+  const uint32_t cz = key / P.nxy;
+  const uint32_t rem = key - cz * P.nxy;
+  const uint32_t cy = rem / P.nx;
+  const uint32_t cx = rem - cy * P.nx;
+  uint32_t *out = list + (size_t)(s >> 6) * kListStride + (s & 63u);
+  int c = 0;
+  uint32_t dropped = 0;
+  const uint32_t x0 = cx > 0 ? cx - 1 : 0;
+  const uint32_t x1 = cx + 1 < P.nx ? cx + 1 : P.nx - 1;
+  for (int dz = -1; dz <= 1; ++dz) {
+    const int z = (int)cz + dz;
+    if (z < 0 || z >= (int)P.nz) continue;
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int y = (int)cy + dy;
+      if (y < 0 || y >= (int)P.ny) continue;
+      const uint32_t row = (uint32_t)z * P.nxy + (uint32_t)y * P.nx;
+      const uint32_t j0 = cstart[row + x0];
+      const uint32_t j1 = cstart[row + x1 + 1];
+      for (uint32_t j = j0; j < j1; ++j) {
+        if (j == s) continue;
+        const float4 q = pos[j];
+        const float ddx = p.x - q.x, ddy = p.y - q.y, ddz = p.z - q.z;
+        const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+        bool hit;
+        if (r2 < P.h2_lo)
+          hit = true;
+        else if (r2 >= P.h2_hi)
+          hit = false;
+        else
+          hit = inside_exact(p.x, p.y, p.z, q.x, q.y, q.z, P.h2d);
+        if (hit) {
+          if (c < kMaxNb) {
+            out[(size_t)c * kWave] = j;
+            ++c;
+          } else {
+            ++dropped;
+          }
+        }
+      }
+    }
+  }
+  cnt[s] = c;
+  add_counter64(stats, STAT_DROPPED, dropped);
+}
+
+// ---------------------------------------------------------------------------
+// Jacobi constraint loop
+// ---------------------------------------------------------------------------
+// Max of a non-negative float over the wave, then one atomic per wave on the
+// float's bit pattern (monotonic for non-negative floats).
+__device__ __forceinline__ void wave_max_to(uint32_t *slot, float v, bool active) {
+  float m = active ? v : 0.0f;
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0 && m > 0.0f) atomicMax(slot, __float_as_uint(m));
+}
+
+__device__ __forceinline__ bool converged(const uint32_t *stats) {
+  return stats[STAT_CONVERGED] != 0u;
+}
+
+// ComputeDensities (particles.c:461-514) and ComputeLambda (particles.c:588-666)
+// in one gather: rho_i needs only r^2, lambda_i needs rho_i (own) and the
+// spiky gradients.  Writes rho and pos.w = lambda.
+// MODE 0: both, 1: density only (rho), 2: lambda only (reads rho).
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_density_lambda(
+    uint32_t n, uint32_t nblocks, float4 *pos, const uint32_t *__restrict__ list,
+    const int *__restrict__ cnt, DevParams P, float *rho_io, uint32_t *stats, int err_slot,
+    int check_converged, int density_to_w) {
+  if (check_converged && converged(stats)) return;
+  const uint32_t s = particle_index(nblocks);
+  const bool active = s < n;
+  float err = 0.0f;
+  if (active) {
+    const float4 p = pos[s];
+    const int c = cnt[s];
+    const uint32_t *L = list_ptr(list, s);
+    float su3 = 0.0f, sgx = 0.0f, sgy = 0.0f, sgz = 0.0f, sc = 0.0f;
+    for (int k = 0; k < c; ++k) {
+      const uint32_t j = L[(size_t)k * kWave];
+      const float4 q = pos[j];
+      const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      if (MODE != 2) {
+        const float u = P.h2 - r2;  // W(r) = norm (h^2 - r^2)^3, 0 for r >= h
+        su3 += u > 0.0f ? u * u * u : 0.0f;
+      }
+      if (MODE != 1) {
+        const float r2c = fmaxf(r2, P.rmin2);
+        const float ir = fast_rsq(r2c);
+        const float hr = P.h - r2c * ir;  // h - r, r floored at 1e-4 h
+        const float g = hr > 0.0f ? hr * hr * ir : 0.0f;  // DelW(r)/r / norm
+        sgx += g * dx;
+        sgy += g * dy;
+        sgz += g * dz;
+        sc += g * g * r2;
+      }
+    }
+    float rho;
+    if (MODE != 2) {
+      rho = P.mW0 + P.mWn * su3;
+      rho_io[s] = rho;
+      err = fmaxf((rho - P.rest_density) * P.inv_rest_density, 0.0f);
+      if (density_to_w) pos[s].w = rho;
+    } else {
+      rho = rho_io[s];
+    }
+    if (MODE != 1) {
+      const float Ci = fmaxf((rho - P.rest_density) * P.inv_rest_density, 0.0f);
+      const float sum_c = P.lam_k2 * (sc + sgx * sgx + sgy * sgy + sgz * sgz);
+      const float lam = -Ci / (sum_c + P.lambda_eps);
+      pos[s].w = lam;
+    }
+  }
+  if (MODE != 2 && err_slot >= 0) wave_max_to(&stats[STAT_ERR0 + err_slot], err, active);
+}
+
+// Trilinear sample with normalised coordinates, texel centres at (i+0.5)/dim
+// and clamp addressing, as tex3D with linear filtering (obstacle_cuda.cu:
+// 174-180) defines it, with exact FP32 weights.
+__device__ __forceinline__ float sdf_sample(const SdfDev &o, float u, float v, float w) {
+  const float cu = u * (float)o.dim[0] - 0.5f;
+  const float cv = v * (float)o.dim[1] - 0.5f;
+  const float cw = w * (float)o.dim[2] - 0.5f;
+  const float fu = floorf(cu), fv = floorf(cv), fw = floorf(cw);
+  const float au = cu - fu, av = cv - fv, aw = cw - fw;
+  const int iu = (int)fu, iv = (int)fv, iw = (int)fw;
+  const int u0 = min(max(iu, 0), o.dim[0] - 1), u1 = min(max(iu + 1, 0), o.dim[0] - 1);
+  const int v0 = min(max(iv, 0), o.dim[1] - 1), v1 = min(max(iv + 1, 0), o.dim[1] - 1);
+  const int w0 = min(max(iw, 0), o.dim[2] - 1), w1 = min(max(iw + 1, 0), o.dim[2] - 1);
+  const size_t sy = (size_t)o.dim[0], sz = (size_t)o.dim[0] * (size_t)o.dim[1];
+  const float *d = o.data;
+  const float c000 = d[u0 + v0 * sy + w0 * sz], c100 = d[u1 + v0 * sy + w0 * sz];
+  const float c010 = d[u0 + v1 * sy + w0 * sz], c110 = d[u1 + v1 * sy + w0 * sz];
+  const float c001 = d[u0 + v0 * sy + w1 * sz], c101 = d[u1 + v0 * sy + w1 * sz];
+  const float c011 = d[u0 + v1 * sy + w1 * sz], c111 = d[u1 + v1 * sy + w1 * sz];
+  const float c00 = c000 + au * (c100 - c000), c10 = c010 + au * (c110 - c010);
+  const float c01 = c001 + au * (c101 - c001), c11 = c011 + au * (c111 - c011);
+  const float c0 = c00 + av * (c10 - c00), c1 = c01 + av * (c11 - c01);
+  return c0 + aw * (c1 - c0);
+}
+
+// CalculateParticleCollisions_kernel (obstacle_cuda.cu:56-116).
+__device__ __forceinline__ void collide(float &x, float &y, float &z, const SdfDev &o) {
+  if (!o.data) return;
+  if (!(x > o.wmin[0] && x < o.wmax[0] && y > o.wmin[1] && y < o.wmax[1] && z > o.wmin[2] &&
+        z < o.wmax[2]))
+    return;
+  const float u = (x - o.wmin[0]) / o.len[0];
+  const float v = (y - o.wmin[1]) / o.len[1];
+  const float w = (z - o.wmin[2]) / o.len[2];
+  const float d = sdf_sample(o, u, v, w);
+  if (!(d < 0.0f)) return;
+  const float gx = sdf_sample(o, u + o.inv_dim[0], v, w) - sdf_sample(o, u - o.inv_dim[0], v, w);
+  const float gy = sdf_sample(o, u, v + o.inv_dim[1], w) - sdf_sample(o, u, v - o.inv_dim[1], w);
+  const float gz = sdf_sample(o, u, v, w + o.inv_dim[2]) - sdf_sample(o, u, v, w - o.inv_dim[2]);
+  const float dd = gx * gx + gy * gy + gz * gz;
+  if (!(dd > 0.0f)) return;  // the reference would normalise a zero vector to NaN
+  const float inv = 1.0f / sqrtf(dd);
+  const float dist = fabsf(d) * o.m2w;
+  y += dist * (gy * inv);
+  x += dist * (gx * inv);
+  z += dist * (gz * inv);
+}
+
+// UpdateDPs (particles.c:668-746): returns sum_j (lambda_i + lambda_j) DelW(r)/r d.
+__device__ __forceinline__ float3 dp_sum(const float4 &p, const float4 *pos, const uint32_t *L,
+                                         int c, const DevParams &P) {
+  float ax = 0.0f, ay = 0.0f, az = 0.0f;
+  for (int k = 0; k < c; ++k) {
+    const uint32_t j = L[(size_t)k * kWave];
+    const float4 q = pos[j];
+    const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    const float r2c = fmaxf(r2, P.rmin2);
+    const float ir = fast_rsq(r2c);
+    const float hr = P.h - r2c * ir;
+    const float g = hr > 0.0f ? hr * hr * ir : 0.0f;
+    const float f = (p.w + q.w) * g;
+    ax += f * dx;
+    ay += f * dy;
+    az += f * dz;
+  }
+  return make_float3(ax, ay, az);
+}
+
+// Staged UpdateDPs: dp = stiffness * m * DelW_norm / rho0 * sum.
+__global__ __launch_bounds__(kBlock) void k_dp(uint32_t n, uint32_t nblocks,
+                                               const float4 *__restrict__ pos,
+                                               const uint32_t *__restrict__ list,
+                                               const int *__restrict__ cnt, DevParams P,
+                                               float scale, float4 *__restrict__ dp) {
+  const uint32_t s = particle_index(nblocks);
+  if (s >= n) return;
+  const float4 p = pos[s];
+  const float3 a = dp_sum(p, pos, list_ptr(list, s), cnt[s], P);
+  dp[s] = make_float4(scale * a.x, scale * a.y, scale * a.z, 0.0f);
+}
+
+// Staged UpdatePositionStars (particles.c:529-564): x* += dp, clamp, collide.
+// Also stores rho in w for the velocity passes.
+__global__ __launch_bounds__(kBlock) void k_position_stars(uint32_t n, float4 *__restrict__ pos,
+                                                           const float4 *__restrict__ dp,
+                                                           const float *__restrict__ rho,
+                                                           DevParams P, SdfDev o) {
+  const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= n) return;
+  const float4 p = pos[s], d = dp[s];
+  float x = p.x + d.x, y = p.y + d.y, z = p.z + d.z;
+  apply_boundary(x, y, z, P);
+  collide(x, y, z, o);
+  pos[s] = make_float4(x, y, z, rho[s]);
+}
+
+// Fused UpdateDPs + UpdatePositionStars: reads pin (x*, lambda), writes pout
+// (x*', rho).  In tolerance mode, once the density pass of this iteration is
+// within tolerance, the particles are copied unchanged instead.
+__global__ __launch_bounds__(kBlock) void k_dp_update(
+    uint32_t n, uint32_t nblocks, const float4 *__restrict__ pin, float4 *__restrict__ pout,
+    const float *__restrict__ rho, const uint32_t *__restrict__ list,
+    const int *__restrict__ cnt, DevParams P, float scale, SdfDev o, uint32_t *stats,
+    int err_slot, float tol) {
+  bool done = false;
+  if (tol > 0.0f) {
+    done = converged(stats) ||
+           __uint_as_float(stats[STAT_ERR0 + err_slot]) <= tol;
+  }
+  const uint32_t s = particle_index(nblocks);
+  if (s >= n) return;
+  const float4 p = pin[s];
+  if (done) {
+    if (s == 0) stats[STAT_CONVERGED] = 1u;
+    pout[s] = make_float4(p.x, p.y, p.z, rho[s]);
+    return;
+  }
+  if (s == 0) stats[STAT_ITERS] += 1u;
+  const float3 a = dp_sum(p, pin, list_ptr(list, s), cnt[s], P);
+  float x = p.x + scale * a.x, y = p.y + scale * a.y, z = p.z + scale * a.z;
+  apply_boundary(x, y, z, P);
+  collide(x, y, z, o);
+  pout[s] = make_float4(x, y, z, rho[s]);
+}
+
+// ---------------------------------------------------------------------------
+// Velocity stages
+// ---------------------------------------------------------------------------
+// UpdateVelocities (particles.c:787-838).
+__device__ __forceinline__ float4 velocity_of(const float4 &p, const float4 &xo,
+                                              const DevParams &P) {
+  float vx = (p.x - xo.x) / P.dt, vy = (p.y - xo.y) / P.dt, vz = (p.z - xo.z) / P.dt;
+  vx = fminf(fmaxf(vx, -P.v_max), P.v_max);
+  vy = fminf(fmaxf(vy, -P.v_max), P.v_max);
+  vz = fminf(fmaxf(vz, -P.v_max), P.v_max);
+  return make_float4(vx, vy, vz, 0.0f);
+}
+
+__global__ __launch_bounds__(kBlock) void k_velocity(uint32_t n, const float4 *__restrict__ pos,
+                                                     const float4 *__restrict__ xold,
+                                                     float4 *__restrict__ vel, DevParams P) {
+  const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= n) return;
+  vel[s] = velocity_of(pos[s], xold[s], P);
+}
+
+// ComputeSurfaceTension colour pass (particles.c:94-140), optionally fused
+// with UpdateVelocities (own particle only).  DelWPoly(r)/r = norm (h^2-r^2)^2
+// with r floored at 1e-4 h.
+template <bool WITH_VELOCITY>
+__global__ __launch_bounds__(kBlock) void k_color(uint32_t n, uint32_t nblocks,
+                                                  const float4 *__restrict__ pos,
+                                                  const float4 *__restrict__ xold,
+                                                  const uint32_t *__restrict__ list,
+                                                  const int *__restrict__ cnt, DevParams P,
+                                                  float4 *__restrict__ vel,
+                                                  float4 *__restrict__ color) {
+  const uint32_t s = particle_index(nblocks);
+  if (s >= n) return;
+  const float4 p = pos[s];
+  if (WITH_VELOCITY) vel[s] = velocity_of(p, xold[s], P);
+  const uint32_t *L = list_ptr(list, s);
+  const int c = cnt[s];
+  float ax = 0.0f, ay = 0.0f, az = 0.0f;
+  for (int k = 0; k < c; ++k) {
+    const uint32_t j = L[(size_t)k * kWave];
+    const float4 q = pos[j];
+    const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+    const float r2c = fmaxf(dx * dx + dy * dy + dz * dz, P.rmin2);
+    const float u = P.h2 - r2c;
+    const float g = u > 0.0f ? u * u * fast_rcp(q.w) : 0.0f;
+    ax += g * dx;
+    ay += g * dy;
+    az += g * dz;
+  }
+  color[s] = make_float4(P.color_k * ax, P.color_k * ay, P.color_k * az, 0.0f);
+}
+
+// ComputeSurfaceTension force pass (particles.c:142-204): cohesion (Akinci
+// spline C, particles.c:54-65) + curvature, v += dt F / rho_i in place.
+__global__ __launch_bounds__(kBlock) void k_surface(uint32_t n, uint32_t nblocks,
+                                                    const float4 *__restrict__ pos,
+                                                    const float4 *__restrict__ color,
+                                                    const uint32_t *__restrict__ list,
+                                                    const int *__restrict__ cnt, DevParams P,
+                                                    float4 *__restrict__ vel) {
+  const uint32_t s = particle_index(nblocks);
+  if (s >= n) return;
+  const float4 p = pos[s];
+  const float4 ci = color[s];
+  const uint32_t *L = list_ptr(list, s);
+  const int c = cnt[s];
+  const float hh = 0.5f * P.h;
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+  for (int k = 0; k < c; ++k) {
+    const uint32_t j = L[(size_t)k * kWave];
+    const float4 q = pos[j];
+    const float4 cq = color[j];
+    const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+    const float r2c = fmaxf(dx * dx + dy * dy + dz * dz, P.rmin2);
+    const float ir = fast_rsq(r2c);
+    const float r = r2c * ir;
+    float spline = 0.0f;
+    if (r <= P.h) {
+      const float t = (P.h - r) * r;
+      const float t3 = t * t * t;
+      spline = r <= hh ? 2.0f * t3 - P.h6_64 : t3;
+    }
+    const float coh = P.coh_k * spline * ir;
+    const float K = 2.0f * P.rest_density * fast_rcp(p.w + q.w);
+    fx += K * (coh * dx + P.curv_k * (ci.x - cq.x));
+    fy += K * (coh * dy + P.curv_k * (ci.y - cq.y));
+    fz += K * (coh * dz + P.curv_k * (ci.z - cq.z));
+  }
+  const float k = P.dt / p.w;
+  float4 v = vel[s];
+  v.x += k * fx;
+  v.y += k * fy;
+  v.z += k * fz;
+  vel[s] = v;
+}
+
+// ApplyViscosity (particles.c:383-459), reading v_old and writing v_new.  The
+// distance keeps the reference's typo r^2 = dx^2 + dy^2 dz^2 (particles.c:439).
+__global__ __launch_bounds__(kBlock) void k_xsph(uint32_t n, uint32_t nblocks,
+                                                 const float4 *__restrict__ pos,
+                                                 const float4 *__restrict__ vin,
+                                                 const uint32_t *__restrict__ list,
+                                                 const int *__restrict__ cnt, DevParams P,
+                                                 float4 *__restrict__ vout) {
+  const uint32_t s = particle_index(nblocks);
+  if (s >= n) return;
+  const float4 p = pos[s];
+  const float4 vi = vin[s];
+  const uint32_t *L = list_ptr(list, s);
+  const int c = cnt[s];
+  float ax = 0.0f, ay = 0.0f, az = 0.0f;
+  for (int k = 0; k < c; ++k) {
+    const uint32_t j = L[(size_t)k * kWave];
+    const float4 q = pos[j];
+    const float4 vq = vin[j];
+    const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+    const float u = P.h2 - (dx * dx + dy * dy * dz * dz);
+    const float w = u > 0.0f ? u * u * u * fast_rcp(q.w) : 0.0f;
+    ax += (vq.x - vi.x) * w;
+    ay += (vq.y - vi.y) * w;
+    az += (vq.z - vi.z) * w;
+  }
+  const float k = P.xsph_c * P.mWn;
+  vout[s] = make_float4(vi.x + k * ax, vi.y + k * ay, vi.z + k * az, 0.0f);
+}
+
+// ComputeVorticity (particles.c:207-283); aux = (omega, |omega|).
+__global__ __launch_bounds__(kBlock) void k_vorticity(uint32_t n, uint32_t nblocks,
+                                                      const float4 *__restrict__ pos,
+                                                      const float4 *__restrict__ vel,
+                                                      const uint32_t *__restrict__ list,
+                                                      const int *__restrict__ cnt, DevParams P,
+                                                      float4 *__restrict__ omega) {
+  const uint32_t s = particle_index(nblocks);
+  if (s >= n) return;
+  const float4 p = pos[s];
+  const float4 vi = vel[s];
+  const uint32_t *L = list_ptr(list, s);
+  const int c = cnt[s];
+  float ox = 0.0f, oy = 0.0f, oz = 0.0f;
+  for (int k = 0; k < c; ++k) {
+    const uint32_t j = L[(size_t)k * kWave];
+    const float4 q = pos[j];
+    const float4 vq = vel[j];
+    const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+    const float r2c = fmaxf(dx * dx + dy * dy + dz * dz, P.rmin2);
+    const float ir = fast_rsq(r2c);
+    const float hr = P.h - r2c * ir;
+    float g = hr > 0.0f ? hr * hr * ir : 0.0f;
+    if (P.vort_model == 1) g *= fast_rcp(q.w);
+    const float wx = g * dx, wy = g * dy, wz = g * dz;
+    const float vx = vq.x - vi.x, vy = vq.y - vi.y, vz = vq.z - vi.z;
+    ox += vy * wz - vz * wy;
+    oy += vz * wx - vx * wz;
+    oz += vx * wy - vy * wx;
+  }
+  const float k = P.vort_model == 1 ? P.DelW_norm * P.mass : P.DelW_norm;
+  ox *= k;
+  oy *= k;
+  oz *= k;
+  omega[s] = make_float4(ox, oy, oz, sqrtf(ox * ox + oy * oy + oz * oz));
+}
+
+// ApplyVorticityConfinement (particles.c:285-381): eta = sum |omega_j| gradW,
+// N = eta / max(|eta|, 1e-4 h), v += dt eps (N x omega_i) / m.  When COMMIT
+// is set the result is also UpdatePositions (particles.c:566-586): x* and v
+// are scattered back to particle-id order.
+template <bool COMMIT>
+__global__ __launch_bounds__(kBlock) void k_confine(
+    uint32_t n, uint32_t nblocks, const float4 *__restrict__ pos,
+    const float4 *__restrict__ omega, const uint32_t *__restrict__ list,
+    const int *__restrict__ cnt, DevParams P, const float4 *__restrict__ vin,
+    float4 *__restrict__ vout, const uint32_t *__restrict__ id_s, float4 *__restrict__ x_can,
+    float4 *__restrict__ v_can) {
+  const uint32_t s = particle_index(nblocks);
+  if (s >= n) return;
+  const float4 p = pos[s];
+  const uint32_t *L = list_ptr(list, s);
+  const int c = cnt[s];
+  float ex = 0.0f, ey = 0.0f, ez = 0.0f;
+  for (int k = 0; k < c; ++k) {
+    const uint32_t j = L[(size_t)k * kWave];
+    const float4 q = pos[j];
+    const float wm = omega[j].w;
+    const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+    const float r2c = fmaxf(dx * dx + dy * dy + dz * dz, P.rmin2);
+    const float ir = fast_rsq(r2c);
+    const float hr = P.h - r2c * ir;
+    const float g = hr > 0.0f ? wm * hr * hr * ir : 0.0f;
+    ex += g * dx;
+    ey += g * dy;
+    ez += g * dz;
+  }
+  ex *= P.DelW_norm;
+  ey *= P.DelW_norm;
+  ez *= P.DelW_norm;
+  const float em = fmaxf(sqrtf(ex * ex + ey * ey + ez * ez), P.rmin);
+  const float Nx = ex / em, Ny = ey / em, Nz = ez / em;
+  const float4 wi = omega[s];
+  float4 v = vin[s];
+  v.x += P.conf_k * (Ny * wi.z - Nz * wi.y);
+  v.y += P.conf_k * (Nz * wi.x - Nx * wi.z);
+  v.z += P.conf_k * (Nx * wi.y - Ny * wi.x);
+  v.w = 0.0f;
+  if (COMMIT) {
+    const uint32_t id = id_s[s];
+    x_can[id] = make_float4(p.x, p.y, p.z, 0.0f);
+    v_can[id] = v;
+  } else {
+    vout[s] = v;
+  }
+}
+
+// UpdatePositions (particles.c:566-586) with the scatter to id order.
+__global__ __launch_bounds__(kBlock) void k_commit(uint32_t n, const float4 *__restrict__ pos,
+                                                   const float4 *__restrict__ vel,
+                                                   const uint32_t *__restrict__ id_s,
+                                                   float4 *__restrict__ x_can,
+                                                   float4 *__restrict__ v_can) {
+  const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= n) return;
+  const uint32_t id = id_s[s];
+  const float4 p = pos[s];
+  const float4 v = vel[s];
+  x_can[id] = make_float4(p.x, p.y, p.z, 0.0f);
+  v_can[id] = make_float4(v.x, v.y, v.z, 0.0f);
+}
+
+// Substep bookkeeping: clears per-iteration error slots and flags.  A kernel
+// rather than hipMemsetAsync so that the substep can be captured in a graph.
+__global__ void k_substep_init(uint32_t *stats) {
+  const int i = threadIdx.x;
+  if (i >= STAT_ITERS && i < STAT_WORDS) stats[i] = 0u;
+}
+
+// Standalone CalculateParticleCollisions on SoA float arrays.
+__global__ __launch_bounds__(kBlock) void k_collide_soa(uint32_t n, float *__restrict__ x,
+                                                        float *__restrict__ y,
+                                                        float *__restrict__ z, SdfDev o) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  float a = x[i], b = y[i], c = z[i];
+  collide(a, b, c, o);
+  x[i] = a;
+  y[i] = b;
+  z[i] = c;
+}
+
+// FP64 kinetic energy: per-block partial sums, then one atomic per block.
+__global__ __launch_bounds__(kBlock) void k_kinetic_energy(uint32_t n,
+                                                           const float4 *__restrict__ v,
+                                                           double half_m, double *out) {
+  __shared__ double part[kBlock / 64];
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  double e = 0.0;
+  if (i < n) {
+    const float4 u = v[i];
+    e = half_m * ((double)u.x * u.x + (double)u.y * u.y + (double)u.z * u.z);
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) e += __shfl_xor(e, o, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = e;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) t += part[w];
+    atomicAdd(out, t);
+  }
+}
+
+__global__ void k_zero_f64(double *p) { *p = 0.0; }
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+#define LAUNCH_TIMED(ctx, kid, grid, ...)                               \
+  do {                                                                  \
+    hipEvent_t ev_ = nullptr;                                           \
+    prof_begin(ctx, kid, &ev_);                                         \
+    __VA_ARGS__;                                                        \
+    prof_end(ctx, kid, ev_);                                            \
+  } while (0)
+
+int launch_gravity(sph_ctx *c) {
+  const uint32_t n = c->n;
+  if (!n) return SPH_OK;
+  LAUNCH_TIMED(c, K_GRAVITY, 0,
+               k_gravity<<<blocks_for(n), kBlock, 0, c->stream>>>(n, c->v_can, c->dev));
+  SPH_HIP(hipGetLastError());
+  return SPH_OK;
+}
+
+int launch_predict(sph_ctx *c) {
+  const uint32_t n = c->n;
+  if (!n) return SPH_OK;
+  LAUNCH_TIMED(c, K_PREDICT, 0,
+               k_predict<<<blocks_for(n), kBlock, 0, c->stream>>>(n, c->x_can, c->v_can,
+                                                                  c->xs_can, c->dev));
+  SPH_HIP(hipGetLastError());
+  return SPH_OK;
+}
+
+// Hash (fused with gravity+predict when `fused`), sort, cell tables, reorder,
+// lists: FindAllNeighbors (neighbors.c:68-75) in sorted space.
+int launch_neighbor_search(sph_ctx *c, bool fused) {
+  const uint32_t n = c->n;
+  hipStream_t s = c->stream;
+  if (n) {
+    if (fused) {
+      LAUNCH_TIMED(c, K_PREDICT_HASH, 0,
+                   k_predict_hash<<<blocks_for(n), kBlock, 0, s>>>(
+                       n, c->x_can, c->v_can, c->xs_can, c->keys, c->cell_count, c->dev,
+                       c->stats));
+    } else {
+      LAUNCH_TIMED(c, K_HASH, 0,
+                   k_hash<<<blocks_for(n), kBlock, 0, s>>>(n, c->xs_can, c->keys,
+                                                           c->cell_count, c->dev, c->stats));
+    }
+    SPH_HIP(hipGetLastError());
+  }
+  // keys -> (key, id) sorted; ids are implicit 0..n-1 on the first pass
+  const uint32_t *ko = nullptr, *vo = nullptr;
+  int rc = radix_sort_pairs(c->keys, nullptr, c->keys_alt, c->ids_alt, c->keys, c->ids, n,
+                            c->key_bits, &c->hist, &c->hist_cap, &c->scan, s, c, &ko, &vo);
+  if (rc) return rc;
+  if (n == 0) {
+    ko = c->keys;
+    vo = c->ids;
+  }
+  c->key_s = ko;
+  c->id_s = vo;
+  // cell_start[c] = #keys < c (lower bound), cell_start[cells] = n; re-zeroes counts
+  rc = exclusive_scan(c->cell_count, c->cell_start, c->cells, true, true, &c->scan, s, c);
+  if (rc) return rc;
+  if (n) {
+    c->pos = c->pos_a;
+    LAUNCH_TIMED(c, K_REORDER, 0,
+                 k_reorder<<<blocks_for(n), kBlock, 0, s>>>(n, c->id_s, c->xs_can, c->x_can,
+                                                            c->pos, c->xold));
+    SPH_HIP(hipGetLastError());
+    const uint32_t nb = blocks_for(n);
+    LAUNCH_TIMED(c, K_BUILD_LISTS, 0,
+                 k_build_lists<<<nb, kBlock, 0, s>>>(n, nb, c->pos, c->key_s, c->cell_start,
+                                                     c->dev, c->list, c->list_cnt, c->stats));
+    SPH_HIP(hipGetLastError());
+  } else {
+    c->pos = c->pos_a;
+  }
+  c->vel = c->vel_a;
+  c->sorted_valid = true;
+  c->lists_valid = true;
+  return SPH_OK;
+}
+
+static float stiffness_scale(const sph_ctx *c, int substep) {
+  // particles.c:693-694, 742-744: stiffness * m * DelW_norm / rho0
+  const double st = 1.0 - pow(1.0 - c->P.stiffness_k, 1.0 / (double)substep);
+  return (float)(st * c->P.rest_mass * (-45.0 / (M_PI * pow(c->P.smoothing_radius, 6.0))) /
+                 c->P.rest_density);
+}
+
+int launch_density(sph_ctx *c, int mode, int err_slot) {
+  const uint32_t n = c->n;
+  if (!n) return SPH_OK;
+  const uint32_t nb = blocks_for(n);
+  hipStream_t s = c->stream;
+  if (mode == 0)
+    LAUNCH_TIMED(c, K_DENSITY_LAMBDA, 0,
+                 k_density_lambda<0><<<nb, kBlock, 0, s>>>(n, nb, c->pos, c->list, c->list_cnt,
+                                                           c->dev, c->rho, c->stats, err_slot,
+                                                           c->P.density_tolerance > 0, 0));
+  else if (mode == 1)
+    LAUNCH_TIMED(c, K_DENSITY, 0,
+                 k_density_lambda<1><<<nb, kBlock, 0, s>>>(n, nb, c->pos, c->list, c->list_cnt,
+                                                           c->dev, c->rho, c->stats, err_slot, 0,
+                                                           0));
+  else if (mode == 3)  // final density of a capped tolerance loop: rho also into w
+    LAUNCH_TIMED(c, K_DENSITY, 0,
+                 k_density_lambda<1><<<nb, kBlock, 0, s>>>(n, nb, c->pos, c->list, c->list_cnt,
+                                                           c->dev, c->rho, c->stats, err_slot, 1,
+                                                           1));
+  else
+    LAUNCH_TIMED(c, K_LAMBDA, 0,
+                 k_density_lambda<2><<<nb, kBlock, 0, s>>>(n, nb, c->pos, c->list, c->list_cnt,
+                                                           c->dev, c->rho, c->stats, -1, 0, 0));
+  SPH_HIP(hipGetLastError());
+  return SPH_OK;
+}
+
+int launch_dp(sph_ctx *c, int substep) {
+  const uint32_t n = c->n;
+  if (!n) return SPH_OK;
+  const uint32_t nb = blocks_for(n);
+  LAUNCH_TIMED(c, K_DP, 0,
+               k_dp<<<nb, kBlock, 0, c->stream>>>(n, nb, c->pos, c->list, c->list_cnt, c->dev,
+                                                  stiffness_scale(c, substep), c->dpbuf));
+  SPH_HIP(hipGetLastError());
+  return SPH_OK;
+}
+
+int launch_position_stars(sph_ctx *c) {
+  const uint32_t n = c->n;
+  if (!n) return SPH_OK;
+  LAUNCH_TIMED(c, K_POS_STARS, 0,
+               k_position_stars<<<blocks_for(n), kBlock, 0, c->stream>>>(
+                   n, c->pos, c->dpbuf, c->rho, c->dev, c->sdf));
+  SPH_HIP(hipGetLastError());
+  return SPH_OK;
+}
+
+int launch_dp_update(sph_ctx *c, int substep, int err_slot) {
+  const uint32_t n = c->n;
+  if (!n) return SPH_OK;
+  const uint32_t nb = blocks_for(n);
+  float4 *out = (c->pos == c->pos_a) ? c->pos_b : c->pos_a;
+  LAUNCH_TIMED(c, K_DP_UPDATE, 0,
+               k_dp_update<<<nb, kBlock, 0, c->stream>>>(
+                   n, nb, c->pos, out, c->rho, c->list, c->list_cnt, c->dev,
+                   stiffness_scale(c, substep), c->sdf, c->stats, err_slot,
+                   (float)c->P.density_tolerance));
+  SPH_HIP(hipGetLastError());
+  c->pos = out;
+  return SPH_OK;
+}
+
+int launch_velocity(sph_ctx *c) {
+  const uint32_t n = c->n;
+  c->vel = c->vel_a;
+  if (!n) return SPH_OK;
+  LAUNCH_TIMED(c, K_VELOCITY, 0,
+               k_velocity<<<blocks_for(n), kBlock, 0, c->stream>>>(n, c->pos, c->xold, c->vel,
+                                                                   c->dev));
+  SPH_HIP(hipGetLastError());
+  return SPH_OK;
+}
+
+int launch_color(sph_ctx *c, bool with_velocity) {
+  const uint32_t n = c->n;
+  if (with_velocity) c->vel = c->vel_a;
+  if (!n) return SPH_OK;
+  const uint32_t nb = blocks_for(n);
+  if (with_velocity)
+    LAUNCH_TIMED(c, K_COLOR, 0,
+                 k_color<true><<<nb, kBlock, 0, c->stream>>>(n, nb, c->pos, c->xold, c->list,
+                                                             c->list_cnt, c->dev, c->vel,
+                                                             c->aux));
+  else
+    LAUNCH_TIMED(c, K_COLOR, 0,
+                 k_color<false><<<nb, kBlock, 0, c->stream>>>(n, nb, c->pos, c->xold, c->list,
+                                                              c->list_cnt, c->dev, c->vel,
+                                                              c->aux));
+  SPH_HIP(hipGetLastError());
+  return SPH_OK;
+}
+
+int launch_surface(sph_ctx *c) {
+  const uint32_t n = c->n;
+  if (!n) return SPH_OK;
+  const uint32_t nb = blocks_for(n);
+  LAUNCH_TIMED(c, K_SURFACE, 0,
+               k_surface<<<nb, kBlock, 0, c->stream>>>(n, nb, c->pos, c->aux, c->list,
+                                                       c->list_cnt, c->dev, c->vel));
+  SPH_HIP(hipGetLastError());
+  return SPH_OK;
+}
+
+int launch_xsph(sph_ctx *c) {
+  const uint32_t n = c->n;
+  float4 *out = (c->vel == c->vel_a) ? c->vel_b : c->vel_a;
+  if (n) {
+    const uint32_t nb = blocks_for(n);
+    LAUNCH_TIMED(c, K_XSPH, 0,
+                 k_xsph<<<nb, kBlock, 0, c->stream>>>(n, nb, c->pos, c->vel, c->list,
+                                                      c->list_cnt, c->dev, out));
+    SPH_HIP(hipGetLastError());
+  }
+  c->vel = out;
+  return SPH_OK;
+}
+
+int launch_vorticity(sph_ctx *c) {
+  const uint32_t n = c->n;
+  if (!n) return SPH_OK;
+  const uint32_t nb = blocks_for(n);
+  LAUNCH_TIMED(c, K_VORTICITY, 0,
+               k_vorticity<<<nb, kBlock, 0, c->stream>>>(n, nb, c->pos, c->vel, c->list,
+                                                         c->list_cnt, c->dev, c->aux));
+  SPH_HIP(hipGetLastError());
+  return SPH_OK;
+}
+
+int launch_confine(sph_ctx *c, bool commit) {
+  const uint32_t n = c->n;
+  float4 *out = (c->vel == c->vel_a) ? c->vel_b : c->vel_a;
+  if (n) {
+    const uint32_t nb = blocks_for(n);
+    if (commit)
+      LAUNCH_TIMED(c, K_CONFINE, 0,
+                   k_confine<true><<<nb, kBlock, 0, c->stream>>>(
+                       n, nb, c->pos, c->aux, c->list, c->list_cnt, c->dev, c->vel, out,
+                       c->id_s, c->x_can, c->v_can));
+    else
+      LAUNCH_TIMED(c, K_CONFINE, 0,
+                   k_confine<false><<<nb, kBlock, 0, c->stream>>>(
+                       n, nb, c->pos, c->aux, c->list, c->list_cnt, c->dev, c->vel, out,
+                       c->id_s, c->x_can, c->v_can));
+    SPH_HIP(hipGetLastError());
+  }
+  if (!commit) c->vel = out;
+  return SPH_OK;
+}
+
+int launch_commit(sph_ctx *c) {
+  const uint32_t n = c->n;
+  if (!n) return SPH_OK;
+  LAUNCH_TIMED(c, K_COMMIT, 0,
+               k_commit<<<blocks_for(n), kBlock, 0, c->stream>>>(n, c->pos, c->vel, c->id_s,
+                                                                 c->x_can, c->v_can));
+  SPH_HIP(hipGetLastError());
+  return SPH_OK;
+}
+
+int launch_substep_init(sph_ctx *c) {
+  k_substep_init<<<1, STAT_WORDS, 0, c->stream>>>(c->stats);
+  SPH_HIP(hipGetLastError());
+  return SPH_OK;
+}
+
+int launch_collide_soa(float *x, float *y, float *z, uint32_t n, const SdfDev &o,
+                       hipStream_t s) {
+  if (!n) return SPH_OK;
+  k_collide_soa<<<blocks_for(n), kBlock, 0, s>>>(n, x, y, z, o);
+  SPH_HIP(hipGetLastError());
+  return SPH_OK;
+}
+
+int launch_kinetic_energy(sph_ctx *c, double *out_dev) {
+  k_zero_f64<<<1, 1, 0, c->stream>>>(out_dev);
+  if (c->n)
+    k_kinetic_energy<<<blocks_for(c->n), kBlock, 0, c->stream>>>(
+        c->n, c->v_can, 0.5 * c->P.rest_mass, out_dev);
+  SPH_HIP(hipGetLastError());
+  return SPH_OK;
+}
+
+}  // namespace sph

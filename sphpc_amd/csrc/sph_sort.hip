@@ -1,0 +1,400 @@
+// Stable key/value radix sort, exclusive scan and cell bounds for gfx950.
+//
+// Replaces the Thrust wrappers of the reference (thrust_c.cu:86-120):
+//  - SortByKey: LSD radix sort, 8-bit digits.  Each pass is an upsweep
+//    (per-tile digit histogram), a scan of the histogram in digit-major
+//    order, and a downsweep that ranks keys inside the tile with wavefront
+//    ballots (peers with the same digit), stages the tile in LDS in digit
+//    order and writes each digit run out contiguously.  Ranks follow input
+//    position (wave, round, lane), so the sort is stable.
+//  - FindLowerBounds/FindUpperBounds: a key histogram followed by an
+//    exclusive scan gives lower_bound(c) for every cell; upper_bound(c) is
+//    lower_bound(c+1).
+#include "sph_internal.h"
+
+namespace sph {
+
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;  // 4096 keys per block
+constexpr int RS_WAVES = RS_THREADS / 64;
+constexpr int SC_THREADS = 256;
+constexpr int SC_ITEMS = 16;
+constexpr int SC_TILE = SC_THREADS * SC_ITEMS;
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  return x;
+}
+
+// Exclusive scan over a 256-thread block; *tot receives the block total.
+__device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t x, uint32_t *wtot,
+                                                        uint32_t *tot) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint32_t inc = wave_incl_scan(x);
+  if (lane == 63) wtot[w] = inc;
+  __syncthreads();
+  uint32_t pre = 0, all = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const uint32_t t = wtot[i];
+    pre += (i < w) ? t : 0u;
+    all += t;
+  }
+  __syncthreads();
+  *tot = all;
+  return pre + inc - x;
+}
+
+// Lanes of the wave holding the same 8-bit digit (valid lanes only).
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid) {
+  uint64_t peers = __ballot(valid);
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const bool bit = (d >> b) & 1u;
+    const uint64_t bb = __ballot(bit);
+    peers &= bit ? bb : ~bb;
+  }
+  return peers;
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  const int lane = threadIdx.x & 63;
+  return lane ? (~0ull >> (64 - lane)) : 0ull;
+}
+
+// --------------------------------------------------------------------------
+// Scan
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(SC_THREADS) void k_scan_tile_sums(const uint32_t *__restrict__ in,
+                                                               uint32_t len,
+                                                               uint32_t *__restrict__ sums) {
+  __shared__ uint32_t wtot[4];
+  const uint32_t base = blockIdx.x * SC_TILE;
+  uint32_t s = 0;
+#pragma unroll
+  for (int r = 0; r < SC_ITEMS; ++r) {
+    const uint32_t i = base + r * SC_THREADS + threadIdx.x;
+    if (i < len) s += in[i];
+  }
+  uint32_t tot;
+  block_excl_scan_256(s, wtot, &tot);
+  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(SC_THREADS) void k_scan_spine(uint32_t *sums, uint32_t tiles) {
+  __shared__ uint32_t wtot[4];
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < tiles; base += SC_THREADS) {
+    const uint32_t i = base + threadIdx.x;
+    const uint32_t x = i < tiles ? sums[i] : 0u;
+    uint32_t tot;
+    const uint32_t e = block_excl_scan_256(x, wtot, &tot);
+    if (i < tiles) sums[i] = carry + e;
+    carry += tot;
+  }
+}
+
+// in and out may alias (in-place scan).
+__global__ __launch_bounds__(SC_THREADS) void k_scan_apply(uint32_t *in, uint32_t *out,
+                                                           uint32_t len,
+                                                           const uint32_t *__restrict__ sums,
+                                                           int zero_in, int write_total) {
+  __shared__ uint32_t wtot[4];
+  const uint32_t base = blockIdx.x * SC_TILE;
+  uint32_t carry = sums[blockIdx.x];
+  for (int r = 0; r < SC_ITEMS; ++r) {
+    const uint32_t i = base + r * SC_THREADS + threadIdx.x;
+    const uint32_t x = i < len ? in[i] : 0u;
+    uint32_t tot;
+    const uint32_t e = block_excl_scan_256(x, wtot, &tot);
+    if (i < len) {
+      out[i] = carry + e;
+      if (zero_in) in[i] = 0u;
+    }
+    carry += tot;
+  }
+  if (write_total && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[len] = carry;
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, uint32_t len) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < len) p[i] = v;
+}
+
+int exclusive_scan(uint32_t *in, uint32_t *out, uint32_t len, bool zero_in, bool write_total,
+                   ScanScratch *sc, hipStream_t s, sph_ctx *prof) {
+  if (len == 0) {
+    if (write_total) {
+      k_fill_u32<<<1, 64, 0, s>>>(out, 0u, 1u);
+      SPH_HIP(hipGetLastError());
+    }
+    return SPH_OK;
+  }
+  const uint32_t tiles = (len + SC_TILE - 1) / SC_TILE;
+  if (sc->tile_cap < tiles) {
+    if (sc->tile_sums) SPH_HIP(hipFree(sc->tile_sums));
+    sc->tile_sums = nullptr;
+    SPH_HIP(hipMalloc(&sc->tile_sums, sizeof(uint32_t) * tiles));
+    sc->tile_cap = tiles;
+  }
+  hipEvent_t ev = nullptr;
+  if (prof) prof_begin(prof, K_SCAN, &ev);
+  k_scan_tile_sums<<<tiles, SC_THREADS, 0, s>>>(in, len, sc->tile_sums);
+  k_scan_spine<<<1, SC_THREADS, 0, s>>>(sc->tile_sums, tiles);
+  k_scan_apply<<<tiles, SC_THREADS, 0, s>>>(in, out, len, sc->tile_sums, zero_in ? 1 : 0,
+                                            write_total ? 1 : 0);
+  if (prof) prof_end(prof, K_SCAN, ev);
+  SPH_HIP(hipGetLastError());
+  return SPH_OK;
+}
+
+// --------------------------------------------------------------------------
+// Radix sort
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(RS_THREADS) void k_radix_upsweep(const uint32_t *__restrict__ keys,
+                                                              uint32_t n, int shift,
+                                                              uint32_t *__restrict__ hist,
+                                                              uint32_t tiles) {
+  __shared__ uint32_t h[RS_WAVES][256];
+  for (int i = threadIdx.x; i < RS_WAVES * 256; i += RS_THREADS) (&h[0][0])[i] = 0u;
+  __syncthreads();
+  const int w = threadIdx.x >> 6;
+  const uint64_t lt = lanemask_lt();
+  volatile uint32_t *hw = &h[w][0];
+  const uint32_t base = blockIdx.x * RS_TILE;
+#pragma unroll 4
+  for (int r = 0; r < RS_ITEMS; ++r) {
+    const uint32_t i = base + r * RS_THREADS + threadIdx.x;
+    const bool valid = i < n;
+    const uint32_t d = valid ? ((keys[i] >> shift) & 255u) : 0u;
+    const uint64_t peers = match_digit(d, valid);
+    // one leader per digit per wave: leaders of one wave touch distinct slots
+    if (valid && (peers & lt) == 0) hw[d] = hw[d] + (uint32_t)__popcll(peers);
+  }
+  __syncthreads();
+  uint32_t s = 0;
+#pragma unroll
+  for (int ww = 0; ww < RS_WAVES; ++ww) s += h[ww][threadIdx.x];
+  hist[(size_t)threadIdx.x * tiles + blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_radix_downsweep(
+    const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+    uint32_t *__restrict__ kout, uint32_t *__restrict__ vout, uint32_t n, int shift,
+    const uint32_t *__restrict__ offs, uint32_t tiles) {
+  __shared__ uint32_t wcnt[RS_WAVES][256];
+  __shared__ uint32_t dstart[256];
+  __shared__ uint32_t goff[256];
+  __shared__ uint32_t wtot[4];
+  __shared__ uint32_t sk[RS_TILE];
+  __shared__ uint32_t sv[RS_TILE];
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  for (int i = tid; i < RS_WAVES * 256; i += RS_THREADS) (&wcnt[0][0])[i] = 0u;
+  __syncthreads();
+  const uint64_t lt = lanemask_lt();
+  const uint32_t tile0 = blockIdx.x * RS_TILE;
+  const uint32_t wbase = tile0 + (uint32_t)w * 64u * RS_ITEMS;
+  uint32_t k[RS_ITEMS], v[RS_ITEMS], rk[RS_ITEMS];
+  volatile uint32_t *wc = &wcnt[w][0];
+#pragma unroll
+  for (int r = 0; r < RS_ITEMS; ++r) {
+    const uint32_t i = wbase + (uint32_t)r * 64u + (uint32_t)lane;
+    const bool valid = i < n;
+    k[r] = valid ? kin[i] : 0u;
+    v[r] = valid ? (vin ? vin[i] : i) : 0u;
+    const uint32_t d = (k[r] >> shift) & 255u;
+    const uint64_t peers = match_digit(d, valid);
+    const uint32_t below = (uint32_t)__popcll(peers & lt);
+    const uint32_t old = valid ? wc[d] : 0u;
+    rk[r] = old + below;
+    if (valid && below == 0) wc[d] = old + (uint32_t)__popcll(peers);
+  }
+  __syncthreads();
+  // Digit `tid`: prefix over waves, then tile-level digit starts.
+  uint32_t run = 0;
+#pragma unroll
+  for (int ww = 0; ww < RS_WAVES; ++ww) {
+    const uint32_t c = wcnt[ww][tid];
+    wcnt[ww][tid] = run;
+    run += c;
+  }
+  uint32_t tot;
+  const uint32_t ex = block_excl_scan_256(run, wtot, &tot);
+  dstart[tid] = ex;
+  goff[tid] = offs[(size_t)tid * tiles + blockIdx.x];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < RS_ITEMS; ++r) {
+    const uint32_t i = wbase + (uint32_t)r * 64u + (uint32_t)lane;
+    if (i < n) {
+      const uint32_t d = (k[r] >> shift) & 255u;
+      const uint32_t lp = dstart[d] + wcnt[w][d] + rk[r];
+      sk[lp] = k[r];
+      sv[lp] = v[r];
+    }
+  }
+  __syncthreads();
+  const uint32_t tile_n = min((uint32_t)RS_TILE, n - tile0);
+  for (uint32_t i = tid; i < tile_n; i += RS_THREADS) {
+    const uint32_t kk = sk[i];
+    const uint32_t d = (kk >> shift) & 255u;
+    const uint32_t o = goff[d] + (i - dstart[d]);
+    kout[o] = kk;
+    vout[o] = sv[i];
+  }
+}
+
+int radix_sort_pairs(const uint32_t *kin, const uint32_t *vin, uint32_t *k0, uint32_t *v0,
+                     uint32_t *k1, uint32_t *v1, uint32_t n, int key_bits, uint32_t **hist,
+                     size_t *hist_cap, ScanScratch *sc, hipStream_t s, sph_ctx *prof,
+                     const uint32_t **kout, const uint32_t **vout) {
+  *kout = kin;
+  *vout = vin;
+  if (n == 0) return SPH_OK;
+  const int passes = key_bits <= 0 ? 1 : (key_bits + 7) / 8;
+  const uint32_t tiles = (n + RS_TILE - 1) / RS_TILE;
+  const size_t need = (size_t)256 * tiles;
+  if (*hist_cap < need) {
+    if (*hist) SPH_HIP(hipFree(*hist));
+    *hist = nullptr;
+    SPH_HIP(hipMalloc(hist, sizeof(uint32_t) * need));
+    *hist_cap = need;
+  }
+  const uint32_t *ck = kin, *cv = vin;
+  for (int p = 0; p < passes; ++p) {
+    uint32_t *ok = (p & 1) ? k1 : k0;
+    uint32_t *ov = (p & 1) ? v1 : v0;
+    const int shift = 8 * p;
+    hipEvent_t ev = nullptr;
+    if (prof) prof_begin(prof, K_RADIX_UP, &ev);
+    k_radix_upsweep<<<tiles, RS_THREADS, 0, s>>>(ck, n, shift, *hist, tiles);
+    if (prof) prof_end(prof, K_RADIX_UP, ev);
+    SPH_HIP(hipGetLastError());
+    int rc = exclusive_scan(*hist, *hist, (uint32_t)need, false, false, sc, s, prof);
+    if (rc) return rc;
+    if (prof) prof_begin(prof, K_RADIX_DOWN, &ev);
+    k_radix_downsweep<<<tiles, RS_THREADS, 0, s>>>(ck, cv, ok, ov, n, shift, *hist, tiles);
+    if (prof) prof_end(prof, K_RADIX_DOWN, ev);
+    SPH_HIP(hipGetLastError());
+    ck = ok;
+    cv = ov;
+  }
+  *kout = ck;
+  *vout = cv;
+  return SPH_OK;
+}
+
+__global__ void k_count_keys(const uint32_t *__restrict__ keys, uint32_t n,
+                             uint32_t *__restrict__ count, uint32_t cells) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const uint32_t k = keys[i];
+    if (k < cells) atomicAdd(&count[k], 1u);
+  }
+}
+
+int count_keys(const uint32_t *keys, uint32_t n, uint32_t *count, uint32_t cells,
+               hipStream_t s) {
+  const uint32_t len = cells + 1;
+  k_fill_u32<<<blocks_for(len), kBlock, 0, s>>>(count, 0u, len);
+  if (n) k_count_keys<<<blocks_for(n), kBlock, 0, s>>>(keys, n, count, cells);
+  SPH_HIP(hipGetLastError());
+  return SPH_OK;
+}
+
+}  // namespace sph
+
+using namespace sph;
+
+// --------------------------------------------------------------------------
+// C ABI: native sort / bounds (replacing thrust_c.cu:86-120)
+// --------------------------------------------------------------------------
+extern "C" int sph_sort_by_key_bits(uint32_t *keys, uint32_t *values, int n, int key_bits,
+                                    void *stream) {
+  if (n < 0 || (n > 0 && (!keys || !values)) || key_bits < 1 || key_bits > 32) {
+    set_error("sph_sort_by_key: invalid argument");
+    return SPH_ERR_ARG;
+  }
+  if (n <= 1) return SPH_OK;
+  hipStream_t s = (hipStream_t)stream;
+  uint32_t *tk = nullptr, *tv = nullptr, *hist = nullptr;
+  size_t hist_cap = 0;
+  ScanScratch sc;
+  int rc = SPH_OK;
+  const uint32_t *kout = nullptr, *vout = nullptr;
+  hipError_t e = hipMalloc(&tk, sizeof(uint32_t) * (size_t)n);
+  if (e == hipSuccess) e = hipMalloc(&tv, sizeof(uint32_t) * (size_t)n);
+  if (e != hipSuccess) {
+    rc = hip_fail(e, "hipMalloc (sort scratch)");
+  } else {
+    rc = radix_sort_pairs(keys, values, tk, tv, keys, values, (uint32_t)n, key_bits, &hist,
+                          &hist_cap, &sc, s, nullptr, &kout, &vout);
+    if (rc == SPH_OK && kout != keys) {
+      e = hipMemcpyAsync(keys, kout, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToDevice, s);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(values, vout, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToDevice,
+                           s);
+      if (e != hipSuccess) rc = hip_fail(e, "hipMemcpyAsync (sort result)");
+    }
+    e = hipStreamSynchronize(s);
+    if (rc == SPH_OK && e != hipSuccess) rc = hip_fail(e, "hipStreamSynchronize (sort)");
+  }
+  if (tk) (void)hipFree(tk);
+  if (tv) (void)hipFree(tv);
+  if (hist) (void)hipFree(hist);
+  if (sc.tile_sums) (void)hipFree(sc.tile_sums);
+  return rc;
+}
+
+extern "C" int sph_sort_by_key(uint32_t *keys, uint32_t *values, int n, void *stream) {
+  return sph_sort_by_key_bits(keys, values, n, 32, stream);
+}
+
+static int bounds_impl(const uint32_t *keys, int n, int cells, uint32_t *out, void *stream,
+                       bool upper) {
+  if (n < 0 || cells < 0 || (cells > 0 && !out) || (n > 0 && !keys)) {
+    set_error("sph_find_bounds: invalid argument");
+    return SPH_ERR_ARG;
+  }
+  if (cells == 0) return SPH_OK;
+  hipStream_t s = (hipStream_t)stream;
+  uint32_t *count = nullptr, *start = nullptr;
+  ScanScratch sc;
+  int rc = SPH_OK;
+  hipError_t e = hipMalloc(&count, sizeof(uint32_t) * ((size_t)cells + 1));
+  if (e == hipSuccess) e = hipMalloc(&start, sizeof(uint32_t) * ((size_t)cells + 1));
+  if (e != hipSuccess) {
+    rc = hip_fail(e, "hipMalloc (bounds scratch)");
+  } else {
+    rc = count_keys(keys, (uint32_t)n, count, (uint32_t)cells, s);
+    if (rc == SPH_OK)
+      rc = exclusive_scan(count, start, (uint32_t)cells, false, true, &sc, s, nullptr);
+    if (rc == SPH_OK) {
+      e = hipMemcpyAsync(out, start + (upper ? 1 : 0), sizeof(uint32_t) * (size_t)cells,
+                         hipMemcpyDeviceToDevice, s);
+      if (e != hipSuccess) rc = hip_fail(e, "hipMemcpyAsync (bounds)");
+    }
+    e = hipStreamSynchronize(s);
+    if (rc == SPH_OK && e != hipSuccess) rc = hip_fail(e, "hipStreamSynchronize (bounds)");
+  }
+  if (count) (void)hipFree(count);
+  if (start) (void)hipFree(start);
+  if (sc.tile_sums) (void)hipFree(sc.tile_sums);
+  return rc;
+}
+
+extern "C" int sph_find_lower_bounds(const uint32_t *keys, int n, int cells, uint32_t *out,
+                                     void *stream) {
+  return bounds_impl(keys, n, cells, out, stream, false);
+}
+
+extern "C" int sph_find_upper_bounds(const uint32_t *keys, int n, int cells, uint32_t *out,
+                                     void *stream) {
+  return bounds_impl(keys, n, cells, out, stream, true);
+}
