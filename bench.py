@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Particle-updates/s of the position-based-fluids substep on one MI355X.
+
+Default workload (the 8M-particle configuration of the headline metric):
+double dam break, two 256x128x128 FP32 lattice blocks (8,388,608 particles),
+spacing s = 0.008 m, seeded +-0.01 s jitter, at the two x-ends of a 6x2x2 m box;
+h = 2s, rho0 = 1000, mass rho0 s^3, dt = 1/300 s, g = 9.81, relaxation 600,
+4 Jacobi iterations, surface tension 0.5, vorticity confinement 0.01
+(m/rho-weighted form), XSPH 0.01, and a torus (R 0.4, r 0.12) united with an
+axis-aligned box, sampled onto a 256^3 FP32 SDF grid.
+
+One step = one substep of the solver over all particles with the state
+resident in HBM.  Prints one JSON line: the throughput, the roofline of the
+dominant kernel (event-timed on the library's stream), Jacobi iterations to
+0.1% density error in tolerance mode, and the FP64 CPU oracle timed on the
+host cores on a bounded sample of the same state (reported baseline only).
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-updates/s @8M; Jacobi iters to 0.1% density err, warm-start vs plain"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per particle of each kernel (FP32, each stream once per
+# pass, 3-vectors 12 B, scalars 4 B, a neighbour list 4(nbar+1) B).
+KERNEL_BYTES = {
+    "density_lambda": lambda L: 20 + L,   # x* 12, list -> rho 4, lambda 4
+    "dp_update": lambda L: 28 + L,        # x* 12, lambda 4, list -> x*' 12
+    "build_lists": lambda L: 12 + L,      # x* 12 -> list
+    "color": lambda L: 64 + L,            # velocity (x*, x -> v) + colour (x*, rho -> n)
+    "surface": lambda L: 52 + L,          # x*, rho, n, v -> v
+    "xsph": lambda L: 40 + L,             # x*, rho, v -> v
+    "vorticity": lambda L: 40 + L,        # x*, v -> omega
+    "confine": lambda L: 68 + L,          # x*, omega, v, id -> x, v (commit)
+}
+
+
+def scene_cfg2(scale=1.0):
+    import sphpc_amd as sa
+    s = float(np.float32(0.008))
+    nx, ny, nz = (max(1, int(round(256 * scale))), max(1, int(round(128 * scale))),
+                  max(1, int(round(128 * scale))))
+    y0 = (2.0 - ny * s) / 2.0
+    blocks = [sa.Block(nx, ny, nz, 0.0, y0, 0.0),
+              sa.Block(nx, ny, nz, 6.0 - nx * s, y0, 0.0)]
+    pos = sa.scene_lattice(blocks, s, jitter=0.01, seed=2025)
+    dims = (256, 256, 256)
+    wmin, wmax = (2.2, 0.2, 0.0), (3.8, 1.8, 1.6)
+    shapes = [sa.torus((3.0, 0.75, 0.12), 0.4, 0.12, axis=2),
+              sa.box_shape((3.0, 1.45, 0.25), (0.25, 0.2, 0.25))]
+    sdf = sa.sdf_sample_shapes(shapes, dims, wmin, wmax)
+    return dict(s=s, box=(6.0, 2.0, 2.0), pos=pos, sdf=sdf, dims=dims, wmin=wmin, wmax=wmax,
+                dx=(wmax[0] - wmin[0]) / dims[0])
+
+
+def gpu_params(sc, iters=4, tol=0.0):
+    import sphpc_amd as sa
+    s = sc["s"]
+    return sa.default_params(2.0 * s, time_step=1.0 / 300.0, g=9.81, k=0.5, c=0.01,
+                             rest_density=1000.0, rest_mass=1000.0 * s ** 3,
+                             lambda_epsilon=600.0, vorticity_epsilon=0.01, vorticity_model=1,
+                             solve_iterations=iters, density_tolerance=tol)
+
+
+def read_traffic(kernel, workload_tag):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, or None."""
+    path = os.path.join(ROOT, "profiles", f"pmc_traffic_{workload_tag}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) >= 2 and parts[0] == kernel and parts[1].startswith("bytes_per_launch="):
+            return float(parts[1].split("=", 1)[1])
+    return None
+
+
+def cpu_baseline(sc, substeps, threads):
+    """FP64 oracle (C, OpenMP) on the same initial state: 1 warm-up substep, then
+    `substeps` timed ones.  Reported baseline only."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    L = orc.oracle()
+    p = orc.make_params(sc["s"], sc["box"], dt=1.0 / 300.0, g=9.81, k=0.5, c=0.01,
+                        lambda_eps=600.0, vort_eps=0.01, vort_model=1, surface=True,
+                        viscosity=True, vorticity=True)
+    ob = orc.obstacle(sc["sdf"], sc["dims"], sc["dx"], sc["wmin"], sc["wmax"])
+    pos = sc["pos"]
+    n = len(pos)
+    cols = [np.ascontiguousarray(pos[:, i], dtype=np.float64) for i in range(3)]
+    arrs = dict(x=cols[0], y=cols[1], z=cols[2], vx=np.zeros(n), vy=np.zeros(n),
+                vz=np.zeros(n), xs=np.zeros(n), ys=np.zeros(n), zs=np.zeros(n),
+                density=np.zeros(n), lambda_=np.zeros(n),
+                nbr=np.zeros(n * orc.MAXN, np.uint32), cnt=np.zeros(n, np.int32))
+    st = orc.OrcState(**{k: v.ctypes.data for k, v in arrs.items()})
+    L.orc_substep(n, C.byref(st), C.byref(p), C.byref(ob), 4, 0.0, None)
+    t0 = time.perf_counter()
+    for _ in range(substeps):
+        L.orc_substep(n, C.byref(st), C.byref(p), C.byref(ob), 4, 0.0, None)
+    dt = time.perf_counter() - t0
+    return dict(value=n * substeps / dt, unit="particle-updates/s", cores=threads, kind="port",
+                sample=f"FP64 C oracle (OpenMP) on the same {n}-particle initial state: "
+                       f"1 warm-up + {substeps} timed substeps ({dt:.1f} s); "
+                       "reported baseline only")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--scale", type=float, default=1.0,
+                    help="block edge scale (1.0 = the 8,388,608-particle workload)")
+    ap.add_argument("--tol-steps", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="only the timed substeps (for profiler runs)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
+              "the multi-GPU slab decomposition is design-only", file=sys.stderr)
+        sys.exit(2)
+
+    import sphpc_amd as sa
+
+    sc = scene_cfg2(args.scale)
+    n = len(sc["pos"])
+    ctx = sa.Context(gpu_params(sc), sc["box"], n)
+    ctx.upload(sc["pos"])
+    ctx.set_obstacle(sc["sdf"], sc["dims"], sc["dx"], sc["wmin"], sc["wmax"])
+
+    ctx.step(args.warmup)
+    ctx.synchronize()
+    # Timed region.  Every kernel launch is bracketed by HIP events recorded on
+    # the library's own stream; their sums give the per-kernel averages.
+    ctx.profile(True)
+    ctx.profile_reset()
+    t0 = time.perf_counter()
+    ctx.step(args.steps)
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    names = sa.kernel_names()
+    times = {}
+    for k, name in enumerate(names):
+        ms, cnt = ctx.profile_read(k)
+        if cnt:
+            times[name] = (ms, cnt)
+    ctx.profile(False)
+    value = n * args.steps / elapsed
+    st = ctx.stats()
+    nbar = st.neighbor_pairs / max(n, 1)
+    out = {
+        "metric": METRIC, "value": value, "unit": "particle-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "double dam break: 2x256x128x128 FP32 lattice blocks, s=0.008 m, "
+                               "6x2x2 m box, torus+box SDF 256^3, 4 Jacobi iterations, "
+                               "surface tension + XSPH + vorticity",
+                   "particles": n, "scale": args.scale, "mean_neighbors": nbar,
+                   "max_density_error_last": st.max_density_error,
+                   "neighbor_overflow": st.neighbor_overflow},
+    }
+    gathers = {k: v for k, v in times.items() if k in KERNEL_BYTES}
+    dom = max(gathers, key=lambda k: gathers[k][0])
+    ms, cnt = gathers[dom]
+    avg_s = ms / cnt / 1e3
+    L = 4.0 * (nbar + 1.0)
+    bytes_launch = KERNEL_BYTES[dom](L) * n
+    achieved = bytes_launch / avg_s / 1e9
+    traffic = read_traffic(dom, "cfg2" if args.scale == 1.0 else f"scale{args.scale}")
+    out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": achieved,
+                       "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                       "traffic": traffic, "algorithmic_bytes_per_launch": bytes_launch,
+                       "avg_launch_ms": avg_s * 1e3, "launches": cnt}
+    out["kernels_ms_per_substep"] = {k: v[0] / args.steps for k, v in
+                                     sorted(times.items(), key=lambda kv: -kv[1][0])}
+    out["kernels_ms_per_substep"]["_sum"] = sum(v[0] for v in times.values()) / args.steps
+    if not args.no_extras:
+        # Jacobi iterations to max density error <= 0.1% (cap 10), plain solver
+        ctx.set_params(gpu_params(sc, iters=10, tol=1e-3))
+        its, errs = [], []
+        for _ in range(args.tol_steps):
+            ctx.step(1)
+            s2 = ctx.stats()
+            its.append(s2.iterations)
+            errs.append(s2.max_density_error)
+        out["jacobi"] = {"tolerance": 1e-3, "cap": 10,
+                         "plain_iters_per_substep": float(np.mean(its)),
+                         "plain_max_density_error": float(np.max(errs)),
+                         "warm_iters_per_substep": None}
+        out["kinetic_energy_J"] = ctx.kinetic_energy()
+    ctx.close()
+    if not args.no_cpu and not args.no_extras:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["cpu_baseline"] = cpu_baseline(sc, args.cpu_steps, threads)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

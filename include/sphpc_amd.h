@@ -92,6 +92,7 @@ typedef struct sph_stats {
                                            the reference, neighbors.c:285-286) */
   unsigned long long cell_clamps;       /* cell coordinates clamped into the grid */
   long long substeps;        /* substeps run on this context */
+  unsigned long long neighbor_pairs;    /* sum of list lengths of the last search */
 } sph_stats;
 
 typedef struct sph_ctx sph_ctx;
@@ -189,6 +190,69 @@ int sph_device_malloc(void **ptr, size_t bytes);
 int sph_device_free(void *ptr);
 int sph_memcpy_htod(void *dst, const void *src, size_t bytes);
 int sph_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+
+/* Plain device-to-device copy kernel of `bytes` (a multiple of 16); used to
+ * calibrate hardware byte counters against a known traffic volume. */
+int sph_device_copy(void *dst, const void *src, size_t bytes, void *stream);
+
+/* --- scene and obstacle inputs (host side) ------------------------------
+ * Fluid blocks: the lattice of ConstructFluidVolume (geometry.c:42-55),
+ * x fastest, centres at origin + (i + 1/2) * spacing; optional seeded
+ * jitter U(-jitter, jitter) * spacing per axis. */
+typedef struct sph_block {
+  int nx, ny, nz;
+  double origin_x, origin_y, origin_z;
+} sph_block;
+/* With x/y/z NULL only *count is set. */
+int sph_scene_lattice(const sph_block *blocks, int nblocks, double spacing, double jitter,
+                      uint64_t seed, float *x, float *y, float *z, int cap, int *count);
+/* Uniform [0,1) draw of the jitter generator (seed, particle, axis). */
+double sph_scene_uniform(uint64_t seed, uint64_t index, int axis);
+
+enum sph_shape_kind { SPH_SHAPE_SPHERE = 0, SPH_SHAPE_TORUS = 1, SPH_SHAPE_BOX = 2 };
+typedef struct sph_shape {
+  int kind;
+  int axis;          /* torus axis (0 x, 1 y, 2 z) */
+  double center[3];
+  double radius;     /* sphere radius; torus ring radius */
+  double radius2;    /* torus tube radius */
+  double half[3];    /* box half extents */
+} sph_shape;
+/* Union of signed distances sampled at the texel centres of a dim grid over
+ * `bounds` (x fastest), the points tex3D filtering reads. */
+int sph_sdf_sample_shapes(const sph_shape *shapes, int nshapes, int dim_x, int dim_y,
+                          int dim_z, const sph_aabb *bounds, float *out);
+/* SDFGen ASCII format (obstacle.c:13-66). read allocates *values (sph_host_free). */
+int sph_sdf_write_sdfgen(const char *path, int dim_x, int dim_y, int dim_z,
+                         const double origin[3], double dx, const float *values);
+int sph_sdf_read_sdfgen(const char *path, int dims[3], double origin[3], double *dx,
+                        float **values);
+void sph_host_free(void *p);
+
+/* sim-config.ini (ReadParameters, input_parser.cc:27-62) plus an optional
+ * [Solver] section for the constants the reference hard-codes. */
+typedef struct sph_run_config {
+  int number_steps;
+  double time_step;
+  int number_particles;
+  double g, c, k;
+  sph_aabb boundary, initial_fluid;
+  int has_obstacle;
+  char sdf_file[256];
+  double obstacle_min_x, obstacle_max_x, obstacle_min_y, obstacle_min_z;
+  int solve_iterations;     /* simulation.c:72 (10) */
+  double lambda_epsilon;    /* particles.c:663 (0.01) */
+  double vorticity_epsilon; /* < 0: 0.1 h (particles.c:289) */
+  double mass_factor;       /* particles.c:1037 (0.991) */
+  double density_tolerance; /* 0: fixed iterations */
+  int vorticity_model;      /* 0: reference */
+  int surface_tension, viscosity, vorticity;  /* 1: on (reference) */
+  double jitter;            /* lattice jitter in units of spacing (0) */
+  unsigned long long seed;
+  int output_every;         /* 0: int(1/(30 dt)) (simulation.c:113); < 0: never */
+} sph_run_config;
+int sph_config_load(const char *path, sph_run_config *cfg);
+const char *sph_config_error(void);
 
 /* --- per-kernel timing (HIP events on the context stream) -------------- */
 int sph_profile_enable(sph_ctx *ctx, int enable);

@@ -55,7 +55,8 @@ class ObstacleDesc(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("particles", C.c_int), ("iterations", C.c_int), ("converged", C.c_int),
                 ("max_density_error", C.c_float), ("neighbor_overflow", C.c_ulonglong),
-                ("cell_clamps", C.c_ulonglong), ("substeps", C.c_longlong)]
+                ("cell_clamps", C.c_ulonglong), ("substeps", C.c_longlong),
+                ("neighbor_pairs", C.c_ulonglong)]
 
 
 _lib = None
@@ -354,6 +355,63 @@ def cell_bounds(keys, cells):
     check(lib().sph_find_lower_bounds(kb.ptr, len(keys), int(cells), lo.ptr, None))
     check(lib().sph_find_upper_bounds(kb.ptr, len(keys), int(cells), hi.ptr, None))
     return lo.to_array(np.uint32, cells), hi.to_array(np.uint32, cells)
+
+
+class Block(C.Structure):
+    """struct sph_block: lattice block of nx*ny*nz particles at origin."""
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("origin_x", C.c_double),
+                ("origin_y", C.c_double), ("origin_z", C.c_double)]
+
+
+SHAPE_SPHERE, SHAPE_TORUS, SHAPE_BOX = 0, 1, 2
+
+
+class Shape(C.Structure):
+    """struct sph_shape: analytic signed-distance primitive."""
+    _fields_ = [("kind", C.c_int), ("axis", C.c_int), ("center", C.c_double * 3),
+                ("radius", C.c_double), ("radius2", C.c_double), ("half", C.c_double * 3)]
+
+
+def sphere(center, radius):
+    return Shape(SHAPE_SPHERE, 0, (C.c_double * 3)(*center), radius, 0.0, (C.c_double * 3)())
+
+
+def torus(center, ring_radius, tube_radius, axis=2):
+    return Shape(SHAPE_TORUS, axis, (C.c_double * 3)(*center), ring_radius, tube_radius,
+                 (C.c_double * 3)())
+
+
+def box_shape(center, half):
+    return Shape(SHAPE_BOX, 0, (C.c_double * 3)(*center), 0.0, 0.0, (C.c_double * 3)(*half))
+
+
+def scene_lattice(blocks, spacing, jitter=0.0, seed=0):
+    """sph_scene_lattice: float32 (n, 3) positions of the given blocks."""
+    L = lib()
+    L.sph_scene_lattice.argtypes = [C.POINTER(Block), C.c_int, C.c_double, C.c_double,
+                                    C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                    C.POINTER(C.c_int)]
+    arr = (Block * len(blocks))(*blocks)
+    n = C.c_int()
+    check(L.sph_scene_lattice(arr, len(blocks), spacing, jitter, seed, None, None, None, 0,
+                              C.byref(n)))
+    cols = [np.zeros(n.value, np.float32) for _ in range(3)]
+    check(L.sph_scene_lattice(arr, len(blocks), spacing, jitter, seed, cols[0].ctypes.data,
+                              cols[1].ctypes.data, cols[2].ctypes.data, n.value, C.byref(n)))
+    return np.stack(cols, 1)
+
+
+def sdf_sample_shapes(shapes, dims, wmin, wmax):
+    """sph_sdf_sample_shapes: union SDF at texel centres, x fastest (float32)."""
+    L = lib()
+    L.sph_sdf_sample_shapes.argtypes = [C.POINTER(Shape), C.c_int, C.c_int, C.c_int, C.c_int,
+                                        C.POINTER(AABB), C.c_void_p]
+    arr = (Shape * len(shapes))(*shapes)
+    out = np.zeros(int(dims[0]) * int(dims[1]) * int(dims[2]), np.float32)
+    b = AABB(wmin[0], wmax[0], wmin[1], wmax[1], wmin[2], wmax[2])
+    check(L.sph_sdf_sample_shapes(arr, len(shapes), int(dims[0]), int(dims[1]), int(dims[2]),
+                                  C.byref(b), out.ctypes.data))
+    return out
 
 
 def collide(pos, sdf, dims, spacing, wmin, wmax):

@@ -155,7 +155,7 @@ static void free_all(sph_ctx *c) {
   void *bufs[] = {c->x_can, c->v_can, c->xs_can, c->keys, c->keys_alt, c->ids, c->ids_alt,
                   c->hist, c->cell_count, c->cell_start, c->pos_a, c->pos_b, c->xold,
                   c->vel_a, c->vel_b, c->aux, c->dpbuf, c->rho, c->list, c->list_cnt,
-                  c->stats, c->red, c->sdf_buf, c->scan.tile_sums};
+                  c->stats, c->pair_partial, c->red, c->sdf_buf, c->scan.tile_sums};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto &p : c->prof.pending) {
@@ -169,6 +169,16 @@ static void free_all(sph_ctx *c) {
 static int require_state(sph_ctx *c) {
   if (!c) return arg_error("context is NULL");
   if (!c->state_valid) return state_error("no particle state uploaded");
+  return SPH_OK;
+}
+
+template <class T>
+static int fetch(sph_ctx *c, const T *dev, size_t count, std::vector<T> *out) {
+  out->resize(count);
+  if (count)
+    SPH_HIP(hipMemcpyAsync(out->data(), dev, sizeof(T) * count, hipMemcpyDeviceToHost,
+                           c->stream));
+  SPH_HIP(hipStreamSynchronize(c->stream));
   return SPH_OK;
 }
 
@@ -278,6 +288,7 @@ extern "C" int sph_create(const sph_params *params, const sph_aabb *boundary, in
       {(void **)&c->list, sizeof(uint32_t) * groups * kListStride},
       {(void **)&c->list_cnt, sizeof(int) * cap},
       {(void **)&c->stats, sizeof(uint32_t) * STAT_WORDS},
+      {(void **)&c->pair_partial, sizeof(uint32_t) * ((cap + kBlock - 1) / kBlock)},
       {(void **)&c->red, sizeof(double) * 4},
   };
   for (auto &a : allocs) {
@@ -591,6 +602,14 @@ extern "C" int sph_get_stats(sph_ctx *c, sph_stats *st) {
   st->cell_clamps =
       (unsigned long long)w[STAT_CLAMPS] | ((unsigned long long)w[STAT_CLAMPS + 1] << 32);
   st->substeps = c->substeps;
+  if (c->pair_blocks) {
+    std::vector<uint32_t> part;
+    int rc = fetch(c, (const uint32_t *)c->pair_partial, c->pair_blocks, &part);
+    if (rc) return rc;
+    unsigned long long sum = 0;
+    for (uint32_t v : part) sum += v;
+    st->neighbor_pairs = sum;
+  }
   return SPH_OK;
 }
 
@@ -608,16 +627,6 @@ extern "C" int sph_compute_energy(sph_ctx *c, double *ke) {
 // --------------------------------------------------------------------------
 // Inspection
 // --------------------------------------------------------------------------
-template <class T>
-static int fetch(sph_ctx *c, const T *dev, size_t count, std::vector<T> *out) {
-  out->resize(count);
-  if (count)
-    SPH_HIP(hipMemcpyAsync(out->data(), dev, sizeof(T) * count, hipMemcpyDeviceToHost,
-                           c->stream));
-  SPH_HIP(hipStreamSynchronize(c->stream));
-  return SPH_OK;
-}
-
 extern "C" int sph_download_field(sph_ctx *c, int field, float *out) {
   int rc = require_lists(c);
   if (rc) return rc;
@@ -796,3 +805,23 @@ extern "C" const char *sph_profile_kernel_name(int kernel) {
 }
 
 extern "C" int sph_profile_kernel_count(void) { return K_COUNT; }
+
+// --------------------------------------------------------------------------
+// Counter calibration: plain 16-byte-vector copy
+// --------------------------------------------------------------------------
+__global__ void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    dst[i] = src[i];
+}
+
+extern "C" int sph_device_copy(void *dst, const void *src, size_t bytes, void *stream) {
+  if (!dst || !src || bytes % 16) return arg_error("sph_device_copy: bytes must be a multiple of 16");
+  const size_t n = bytes / 16;
+  if (!n) return SPH_OK;
+  const unsigned blocks = (unsigned)std::min<size_t>((n + kBlock - 1) / kBlock, 65536);
+  k_copy16<<<blocks, kBlock, 0, (hipStream_t)stream>>>((const uint4 *)src, (uint4 *)dst, n);
+  SPH_HIP(hipGetLastError());
+  SPH_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return SPH_OK;
+}

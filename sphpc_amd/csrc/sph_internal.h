@@ -118,6 +118,8 @@ struct sph_ctx {
   uint32_t *list = nullptr;
   int *list_cnt = nullptr;
   uint32_t *stats = nullptr;  // sph::STAT_WORDS words
+  uint32_t *pair_partial = nullptr;  // per-block list-length sums of the last search
+  uint32_t pair_blocks = 0;
   double *red = nullptr;      // FP64 reduction scratch
   int last_err_slot = -1;
   int staged_slot = 0;

@@ -34,6 +34,33 @@ __device__ __forceinline__ const uint32_t *list_ptr(const uint32_t *list, uint32
   return list + (size_t)(s >> 6) * kListStride + (s & 63u);
 }
 
+// Visits the c neighbours of a particle in list order.  Each round first
+// issues kBatch list loads, then the kBatch dependent gathers, then the
+// arithmetic, so every wave keeps kBatch loads in flight instead of one
+// dependent chain per neighbour.  Padding slots load the particle itself
+// (`self`, always a valid index) and are skipped by the body.
+constexpr int kBatch = 8;
+
+template <typename T, typename Load, typename Body>
+__device__ __forceinline__ void for_neighbors(const uint32_t *L, int c, uint32_t self, Load load,
+                                              Body body) {
+  for (int k0 = 0; k0 < c; k0 += kBatch) {
+    uint32_t j[kBatch];
+#pragma unroll
+    for (int b = 0; b < kBatch; ++b) j[b] = (k0 + b < c) ? L[(size_t)(k0 + b) * kWave] : self;
+    T q[kBatch];
+#pragma unroll
+    for (int b = 0; b < kBatch; ++b) q[b] = load(j[b]);
+#pragma unroll
+    for (int b = 0; b < kBatch; ++b)
+      if (k0 + b < c) body(q[b]);
+  }
+}
+
+struct Pair4 {
+  float4 a, b;
+};
+
 __device__ __forceinline__ float fast_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
@@ -84,6 +111,23 @@ __device__ __forceinline__ uint32_t hash_key(float x, float y, float z, const De
 
 __device__ __forceinline__ void add_counter64(uint32_t *stats, int slot, uint32_t v) {
   if (v) atomicAdd(reinterpret_cast<unsigned long long *>(stats + slot), (unsigned long long)v);
+}
+
+// Block sum of v written to partial[blockIdx.x] (no atomics; the host adds the
+// partials when it reads the statistics).  All threads of the block call it.
+__device__ __forceinline__ void block_sum_to(uint32_t *partial, uint32_t v) {
+  __shared__ uint32_t wsum[kBlock / kWave];
+  uint32_t t = v;
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) t += __shfl_xor(t, o, 64);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t s = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / kWave; ++w) s += wsum[w];
+    partial[blockIdx.x] = s;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -185,54 +229,64 @@ __device__ __noinline__ bool inside_exact(float px, float py, float pz, float qx
 __global__ __launch_bounds__(kBlock) void k_build_lists(
     uint32_t n, uint32_t nblocks, const float4 *__restrict__ pos,
     const uint32_t *__restrict__ key_s, const uint32_t *__restrict__ cstart, DevParams P,
-    uint32_t *__restrict__ list, int *__restrict__ cnt, uint32_t *__restrict__ stats) {
+    uint32_t *__restrict__ list, int *__restrict__ cnt, uint32_t *__restrict__ stats,
+    uint32_t *__restrict__ pair_partial) {
   const uint32_t s = particle_index(nblocks);
-  if (s >= n) return;
-  const float4 p = pos[s];
-  const uint32_t key = key_s[s];
-  const uint32_t cz = key / P.nxy;
-  const uint32_t rem = key - cz * P.nxy;
-  const uint32_t cy = rem / P.nx;
-  const uint32_t cx = rem - cy * P.nx;
-  uint32_t *out = list + (size_t)(s >> 6) * kListStride + (s & 63u);
   int c = 0;
   uint32_t dropped = 0;
-  const uint32_t x0 = cx > 0 ? cx - 1 : 0;
-  const uint32_t x1 = cx + 1 < P.nx ? cx + 1 : P.nx - 1;
-  for (int dz = -1; dz <= 1; ++dz) {
-    const int z = (int)cz + dz;
-    if (z < 0 || z >= (int)P.nz) continue;
-    for (int dy = -1; dy <= 1; ++dy) {
-      const int y = (int)cy + dy;
-      if (y < 0 || y >= (int)P.ny) continue;
-      const uint32_t row = (uint32_t)z * P.nxy + (uint32_t)y * P.nx;
-      const uint32_t j0 = cstart[row + x0];
-      const uint32_t j1 = cstart[row + x1 + 1];
-      for (uint32_t j = j0; j < j1; ++j) {
-        if (j == s) continue;
-        const float4 q = pos[j];
-        const float ddx = p.x - q.x, ddy = p.y - q.y, ddz = p.z - q.z;
-        const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-        bool hit;
-        if (r2 < P.h2_lo)
-          hit = true;
-        else if (r2 >= P.h2_hi)
-          hit = false;
-        else
-          hit = inside_exact(p.x, p.y, p.z, q.x, q.y, q.z, P.h2d);
-        if (hit) {
-          if (c < kMaxNb) {
-            out[(size_t)c * kWave] = j;
-            ++c;
-          } else {
-            ++dropped;
+  if (s < n) {
+    const float4 p = pos[s];
+    const uint32_t key = key_s[s];
+    const uint32_t cz = key / P.nxy;
+    const uint32_t rem = key - cz * P.nxy;
+    const uint32_t cy = rem / P.nx;
+    const uint32_t cx = rem - cy * P.nx;
+    uint32_t *out = list + (size_t)(s >> 6) * kListStride + (s & 63u);
+    const uint32_t x0 = cx > 0 ? cx - 1 : 0;
+    const uint32_t x1 = cx + 1 < P.nx ? cx + 1 : P.nx - 1;
+    for (int dz = -1; dz <= 1; ++dz) {
+      const int z = (int)cz + dz;
+      if (z < 0 || z >= (int)P.nz) continue;
+      for (int dy = -1; dy <= 1; ++dy) {
+        const int y = (int)cy + dy;
+        if (y < 0 || y >= (int)P.ny) continue;
+        const uint32_t row = (uint32_t)z * P.nxy + (uint32_t)y * P.nx;
+        const uint32_t j0 = cstart[row + x0];
+        const uint32_t j1 = cstart[row + x1 + 1];
+        // candidates in batches of kBatch loads; hits are appended in j order
+        for (uint32_t jb = j0; jb < j1; jb += kBatch) {
+          float4 q[kBatch];
+#pragma unroll
+          for (int b = 0; b < kBatch; ++b) q[b] = pos[(jb + b < j1) ? jb + b : s];
+#pragma unroll
+          for (int b = 0; b < kBatch; ++b) {
+            const uint32_t j = jb + b;
+            if (j >= j1 || j == s) continue;
+            const float ddx = p.x - q[b].x, ddy = p.y - q[b].y, ddz = p.z - q[b].z;
+            const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+            bool hit;
+            if (r2 < P.h2_lo)
+              hit = true;
+            else if (r2 >= P.h2_hi)
+              hit = false;
+            else
+              hit = inside_exact(p.x, p.y, p.z, q[b].x, q[b].y, q[b].z, P.h2d);
+            if (hit) {
+              if (c < kMaxNb) {
+                out[(size_t)c * kWave] = j;
+                ++c;
+              } else {
+                ++dropped;
+              }
+            }
           }
         }
       }
     }
+    cnt[s] = c;
   }
-  cnt[s] = c;
   add_counter64(stats, STAT_DROPPED, dropped);
+  block_sum_to(pair_partial, (uint32_t)c);
 }
 
 // ---------------------------------------------------------------------------
@@ -240,11 +294,17 @@ __global__ __launch_bounds__(kBlock) void k_build_lists(
 // ---------------------------------------------------------------------------
 // Max of a non-negative float over the wave, then one atomic per wave on the
 // float's bit pattern (monotonic for non-negative floats).
+// The slot only grows during a launch, so a wave whose maximum does not exceed
+// the value it reads skips the atomic; without this every wave would issue a
+// device-scope atomic to one address.
 __device__ __forceinline__ void wave_max_to(uint32_t *slot, float v, bool active) {
   float m = active ? v : 0.0f;
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
-  if ((threadIdx.x & 63) == 0 && m > 0.0f) atomicMax(slot, __float_as_uint(m));
+  if ((threadIdx.x & 63) == 0 && m > 0.0f) {
+    const uint32_t bits = __float_as_uint(m);
+    if (bits > *(volatile uint32_t *)slot) atomicMax(slot, bits);
+  }
 }
 
 __device__ __forceinline__ bool converged(const uint32_t *stats) {
@@ -269,9 +329,7 @@ __global__ __launch_bounds__(kBlock) void k_density_lambda(
     const int c = cnt[s];
     const uint32_t *L = list_ptr(list, s);
     float su3 = 0.0f, sgx = 0.0f, sgy = 0.0f, sgz = 0.0f, sc = 0.0f;
-    for (int k = 0; k < c; ++k) {
-      const uint32_t j = L[(size_t)k * kWave];
-      const float4 q = pos[j];
+    for_neighbors<float4>(L, c, s, [&](uint32_t j) { return pos[j]; }, [&](const float4 &q) {
       const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
       const float r2 = dx * dx + dy * dy + dz * dz;
       if (MODE != 2) {
@@ -288,7 +346,7 @@ __global__ __launch_bounds__(kBlock) void k_density_lambda(
         sgz += g * dz;
         sc += g * g * r2;
       }
-    }
+    });
     float rho;
     if (MODE != 2) {
       rho = P.mW0 + P.mWn * su3;
@@ -358,11 +416,9 @@ __device__ __forceinline__ void collide(float &x, float &y, float &z, const SdfD
 
 // UpdateDPs (particles.c:668-746): returns sum_j (lambda_i + lambda_j) DelW(r)/r d.
 __device__ __forceinline__ float3 dp_sum(const float4 &p, const float4 *pos, const uint32_t *L,
-                                         int c, const DevParams &P) {
+                                         int c, uint32_t self, const DevParams &P) {
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
-  for (int k = 0; k < c; ++k) {
-    const uint32_t j = L[(size_t)k * kWave];
-    const float4 q = pos[j];
+  for_neighbors<float4>(L, c, self, [&](uint32_t j) { return pos[j]; }, [&](const float4 &q) {
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     const float r2 = dx * dx + dy * dy + dz * dz;
     const float r2c = fmaxf(r2, P.rmin2);
@@ -373,7 +429,7 @@ __device__ __forceinline__ float3 dp_sum(const float4 &p, const float4 *pos, con
     ax += f * dx;
     ay += f * dy;
     az += f * dz;
-  }
+  });
   return make_float3(ax, ay, az);
 }
 
@@ -386,7 +442,7 @@ __global__ __launch_bounds__(kBlock) void k_dp(uint32_t n, uint32_t nblocks,
   const uint32_t s = particle_index(nblocks);
   if (s >= n) return;
   const float4 p = pos[s];
-  const float3 a = dp_sum(p, pos, list_ptr(list, s), cnt[s], P);
+  const float3 a = dp_sum(p, pos, list_ptr(list, s), cnt[s], s, P);
   dp[s] = make_float4(scale * a.x, scale * a.y, scale * a.z, 0.0f);
 }
 
@@ -427,7 +483,7 @@ __global__ __launch_bounds__(kBlock) void k_dp_update(
     return;
   }
   if (s == 0) stats[STAT_ITERS] += 1u;
-  const float3 a = dp_sum(p, pin, list_ptr(list, s), cnt[s], P);
+  const float3 a = dp_sum(p, pin, list_ptr(list, s), cnt[s], s, P);
   float x = p.x + scale * a.x, y = p.y + scale * a.y, z = p.z + scale * a.z;
   apply_boundary(x, y, z, P);
   collide(x, y, z, o);
@@ -473,9 +529,7 @@ __global__ __launch_bounds__(kBlock) void k_color(uint32_t n, uint32_t nblocks,
   const uint32_t *L = list_ptr(list, s);
   const int c = cnt[s];
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
-  for (int k = 0; k < c; ++k) {
-    const uint32_t j = L[(size_t)k * kWave];
-    const float4 q = pos[j];
+  for_neighbors<float4>(L, c, s, [&](uint32_t j) { return pos[j]; }, [&](const float4 &q) {
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     const float r2c = fmaxf(dx * dx + dy * dy + dz * dz, P.rmin2);
     const float u = P.h2 - r2c;
@@ -483,7 +537,7 @@ __global__ __launch_bounds__(kBlock) void k_color(uint32_t n, uint32_t nblocks,
     ax += g * dx;
     ay += g * dy;
     az += g * dz;
-  }
+  });
   color[s] = make_float4(P.color_k * ax, P.color_k * ay, P.color_k * az, 0.0f);
 }
 
@@ -503,10 +557,9 @@ __global__ __launch_bounds__(kBlock) void k_surface(uint32_t n, uint32_t nblocks
   const int c = cnt[s];
   const float hh = 0.5f * P.h;
   float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-  for (int k = 0; k < c; ++k) {
-    const uint32_t j = L[(size_t)k * kWave];
-    const float4 q = pos[j];
-    const float4 cq = color[j];
+  auto load = [&](uint32_t j) { return Pair4{pos[j], color[j]}; };
+  for_neighbors<Pair4>(L, c, s, load, [&](const Pair4 &qc) {
+    const float4 &q = qc.a, &cq = qc.b;
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     const float r2c = fmaxf(dx * dx + dy * dy + dz * dz, P.rmin2);
     const float ir = fast_rsq(r2c);
@@ -522,7 +575,7 @@ __global__ __launch_bounds__(kBlock) void k_surface(uint32_t n, uint32_t nblocks
     fx += K * (coh * dx + P.curv_k * (ci.x - cq.x));
     fy += K * (coh * dy + P.curv_k * (ci.y - cq.y));
     fz += K * (coh * dz + P.curv_k * (ci.z - cq.z));
-  }
+  });
   const float k = P.dt / p.w;
   float4 v = vel[s];
   v.x += k * fx;
@@ -546,17 +599,16 @@ __global__ __launch_bounds__(kBlock) void k_xsph(uint32_t n, uint32_t nblocks,
   const uint32_t *L = list_ptr(list, s);
   const int c = cnt[s];
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
-  for (int k = 0; k < c; ++k) {
-    const uint32_t j = L[(size_t)k * kWave];
-    const float4 q = pos[j];
-    const float4 vq = vin[j];
+  auto load = [&](uint32_t j) { return Pair4{pos[j], vin[j]}; };
+  for_neighbors<Pair4>(L, c, s, load, [&](const Pair4 &qv) {
+    const float4 &q = qv.a, &vq = qv.b;
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     const float u = P.h2 - (dx * dx + dy * dy * dz * dz);
     const float w = u > 0.0f ? u * u * u * fast_rcp(q.w) : 0.0f;
     ax += (vq.x - vi.x) * w;
     ay += (vq.y - vi.y) * w;
     az += (vq.z - vi.z) * w;
-  }
+  });
   const float k = P.xsph_c * P.mWn;
   vout[s] = make_float4(vi.x + k * ax, vi.y + k * ay, vi.z + k * az, 0.0f);
 }
@@ -575,10 +627,9 @@ __global__ __launch_bounds__(kBlock) void k_vorticity(uint32_t n, uint32_t nbloc
   const uint32_t *L = list_ptr(list, s);
   const int c = cnt[s];
   float ox = 0.0f, oy = 0.0f, oz = 0.0f;
-  for (int k = 0; k < c; ++k) {
-    const uint32_t j = L[(size_t)k * kWave];
-    const float4 q = pos[j];
-    const float4 vq = vel[j];
+  auto load = [&](uint32_t j) { return Pair4{pos[j], vel[j]}; };
+  for_neighbors<Pair4>(L, c, s, load, [&](const Pair4 &qv) {
+    const float4 &q = qv.a, &vq = qv.b;
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     const float r2c = fmaxf(dx * dx + dy * dy + dz * dz, P.rmin2);
     const float ir = fast_rsq(r2c);
@@ -590,7 +641,7 @@ __global__ __launch_bounds__(kBlock) void k_vorticity(uint32_t n, uint32_t nbloc
     ox += vy * wz - vz * wy;
     oy += vz * wx - vx * wz;
     oz += vx * wy - vy * wx;
-  }
+  });
   const float k = P.vort_model == 1 ? P.DelW_norm * P.mass : P.DelW_norm;
   ox *= k;
   oy *= k;
@@ -615,19 +666,21 @@ __global__ __launch_bounds__(kBlock) void k_confine(
   const uint32_t *L = list_ptr(list, s);
   const int c = cnt[s];
   float ex = 0.0f, ey = 0.0f, ez = 0.0f;
-  for (int k = 0; k < c; ++k) {
-    const uint32_t j = L[(size_t)k * kWave];
+  // only |omega_j| (omega.w) of the neighbour is needed
+  auto load = [&](uint32_t j) {
     const float4 q = pos[j];
-    const float wm = omega[j].w;
+    return make_float4(q.x, q.y, q.z, omega[j].w);
+  };
+  for_neighbors<float4>(L, c, s, load, [&](const float4 &q) {
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     const float r2c = fmaxf(dx * dx + dy * dy + dz * dz, P.rmin2);
     const float ir = fast_rsq(r2c);
     const float hr = P.h - r2c * ir;
-    const float g = hr > 0.0f ? wm * hr * hr * ir : 0.0f;
+    const float g = hr > 0.0f ? q.w * hr * hr * ir : 0.0f;
     ex += g * dx;
     ey += g * dy;
     ez += g * dz;
-  }
+  });
   ex *= P.DelW_norm;
   ey *= P.DelW_norm;
   ez *= P.DelW_norm;
@@ -778,7 +831,9 @@ int launch_neighbor_search(sph_ctx *c, bool fused) {
     const uint32_t nb = blocks_for(n);
     LAUNCH_TIMED(c, K_BUILD_LISTS, 0,
                  k_build_lists<<<nb, kBlock, 0, s>>>(n, nb, c->pos, c->key_s, c->cell_start,
-                                                     c->dev, c->list, c->list_cnt, c->stats));
+                                                     c->dev, c->list, c->list_cnt, c->stats,
+                                                     c->pair_partial));
+    c->pair_blocks = nb;
     SPH_HIP(hipGetLastError());
   } else {
     c->pos = c->pos_a;

@@ -24,6 +24,18 @@ def _soa(a):
     return [np.ascontiguousarray(a[:, i]) for i in range(3)]
 
 
+def _increment_err(v_before, v_after_gpu, v_after_64):
+    """Error of a velocity increment stored as v + dv in FP32: returns the
+    normwise relative error of dv and the bound 1e-5 max|dv| + 2 ulp(max|v|)
+    (the FP32 rounding of v + dv alone is up to half an ulp of |v|)."""
+    d_gpu = d64(v_after_gpu) - d64(v_before)
+    d_64 = d64(v_after_64) - d64(v_before)
+    abs_err = float(np.max(np.abs(d_gpu - d_64)))
+    bound = 1e-5 * float(np.max(np.abs(d_64))) + \
+        2.0 * float(np.spacing(np.float32(np.max(np.abs(d64(v_after_64))))))
+    return abs_err, bound, rel_err(d_gpu, d_64)
+
+
 # ---------------------------------------------------------------------------
 # Native sub-boundary: sort and bounds (thrust_c.cu:86-120)
 # ---------------------------------------------------------------------------
@@ -212,29 +224,30 @@ def test_velocity_passes_match_oracle():
     vv = _soa(v1)
     cg = _soa(col)
     L.orc_apply_surface_force(n, xs, ys, zs, rho, *cg, nbr, cnt, C.byref(sc.o), *vv)
-    e_surf = rel_err(d64(v2) - d64(v1), np.stack(vv, 1) - d64(v1))
-    assert e_surf <= 1e-4, e_surf
+    a_surf, b_surf, e_surf = _increment_err(v1, v2, np.stack(vv, 1))
+    assert a_surf <= b_surf, (a_surf, b_surf)
 
     ctx.apply_viscosity()
     v3 = ctx.field(sa.FIELD_VELOCITY)
     ov = [np.zeros(n) for _ in range(3)]
     L.orc_apply_viscosity(n, xs, ys, zs, rho, *_soa(v2), nbr, cnt, C.byref(sc.o), *ov)
-    e_xsph = rel_err(d64(v3) - d64(v2), np.stack(ov, 1) - d64(v2))
-    assert e_xsph <= 1e-4, e_xsph
+    a_xsph, b_xsph, e_xsph = _increment_err(v2, v3, np.stack(ov, 1))
+    assert a_xsph <= b_xsph, (a_xsph, b_xsph)
 
     ctx.compute_vorticity()
     w = ctx.field(sa.FIELD_VORTICITY)
     w64 = [np.zeros(n) for _ in range(3)]
     L.orc_compute_vorticity(n, xs, ys, zs, rho, *_soa(v3), nbr, cnt, C.byref(sc.o), *w64)
     e_w = rel_err(w, np.stack(w64, 1))
-    assert e_w <= 1e-4, e_w
+    # omega sums (v_j - v_i) x gradW: FP32 velocity differences, normwise 1e-5
+    assert e_w <= 1e-5, e_w
 
     ctx.apply_vorticity_confinement()
     v4 = ctx.field(sa.FIELD_VELOCITY)
     vc = _soa(v3)
     L.orc_apply_vorticity_confinement(n, xs, ys, zs, *_soa(w), nbr, cnt, C.byref(sc.o), *vc)
-    e_conf = rel_err(d64(v4) - d64(v3), np.stack(vc, 1) - d64(v3))
-    assert e_conf <= 1e-4, e_conf
+    a_conf, b_conf, e_conf = _increment_err(v3, v4, np.stack(vc, 1))
+    assert a_conf <= b_conf, (a_conf, b_conf)
 
     ctx.update_positions()
     xg, vg = ctx.download()
@@ -290,8 +303,10 @@ def test_substep_matches_oracle(fused):
 
 
 def test_fused_equals_staged_closely():
+    # vorticity model 1: the reference formula drives |v| to ~1e3 m/s at this
+    # scale in one substep, which only magnifies FP32 ordering differences
     sc, pos, vel = _scene_cfg0(n=(20, 20, 16), seed=6, lambda_eps=600.0, k=0.5, c=0.01,
-                               vort_eps=0.01, iters=3)
+                               vort_eps=0.01, iters=3, vort_model=1)
     n = len(pos)
     out = []
     for fused in (True, False):
@@ -373,5 +388,7 @@ def test_collision_entry_point_matches_oracle():
     pushed = np.any(ref != d64(pts), axis=1)
     record("collision", abs_err=e, pushed=int(pushed.sum()))
     assert pushed.sum() > 1000
-    # FP32 texture coordinates vs the oracle's double ones: ~1e-6 of a texel
-    assert e <= 1e-5 * dx + 4 * np.spacing(np.float32(1.0)), e
+    # The push is |d| * n with n from central differences of FP32 samples:
+    # relative error ~1e-6 of the push distance, plus the FP32 rounding of x.
+    push = float(np.max(np.linalg.norm(ref - d64(pts), axis=1)))
+    assert e <= 1e-5 * push + 4 * np.spacing(np.float32(1.0)), (e, push)
