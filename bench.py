@@ -125,6 +125,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the timed substeps (for profiler runs)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run two 1 GiB device copies (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
@@ -132,6 +134,15 @@ def main():
         sys.exit(2)
 
     import sphpc_amd as sa
+
+    if args.calibrate_copy:
+        # a plain copy of known size, so hardware byte counters can be scaled
+        nbytes = 1 << 30
+        src, dst = sa.DeviceBuffer(nbytes), sa.DeviceBuffer(nbytes)
+        for _ in range(2):
+            sa.check(sa.lib().sph_device_copy(dst.ptr, src.ptr, nbytes, None))
+        src.free()
+        dst.free()
 
     sc = scene_cfg2(args.scale)
     n = len(sc["pos"])

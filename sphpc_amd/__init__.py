@@ -109,6 +109,7 @@ def lib():
     L.sph_calculate_particle_collisions.argtypes = [vp, vp, vp, ip, vp, vp]
     L.sph_device_malloc.argtypes = [C.POINTER(vp), C.c_size_t]
     L.sph_device_free.argtypes = [vp]
+    L.sph_device_copy.argtypes = [vp, vp, C.c_size_t, vp]
     L.sph_memcpy_htod.argtypes = [vp, vp, C.c_size_t]
     L.sph_memcpy_dtoh.argtypes = [vp, vp, C.c_size_t]
     L.sph_get_device_count.argtypes = [C.POINTER(ip)]
