@@ -180,7 +180,8 @@ def main():
                                "surface tension + XSPH + vorticity",
                    "particles": n, "scale": args.scale, "mean_neighbors": nbar,
                    "max_density_error_last": st.max_density_error,
-                   "neighbor_overflow": st.neighbor_overflow},
+                   "neighbor_overflow": st.neighbor_overflow,
+                   "tiles": st.tiles, "tiles_global": st.tiles_global},
     }
     gathers = {k: v for k, v in times.items() if k in KERNEL_BYTES}
     dom = max(gathers, key=lambda k: gathers[k][0])

@@ -93,6 +93,8 @@ typedef struct sph_stats {
   unsigned long long cell_clamps;       /* cell coordinates clamped into the grid */
   long long substeps;        /* substeps run on this context */
   unsigned long long neighbor_pairs;    /* sum of list lengths of the last search */
+  int tiles;                 /* neighbour tiles of the last search */
+  int tiles_global;          /* tiles whose neighbourhood did not fit in LDS */
 } sph_stats;
 
 typedef struct sph_ctx sph_ctx;

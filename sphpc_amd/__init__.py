@@ -56,7 +56,8 @@ class Stats(C.Structure):
     _fields_ = [("particles", C.c_int), ("iterations", C.c_int), ("converged", C.c_int),
                 ("max_density_error", C.c_float), ("neighbor_overflow", C.c_ulonglong),
                 ("cell_clamps", C.c_ulonglong), ("substeps", C.c_longlong),
-                ("neighbor_pairs", C.c_ulonglong)]
+                ("neighbor_pairs", C.c_ulonglong), ("tiles", C.c_int),
+                ("tiles_global", C.c_int)]
 
 
 _lib = None

@@ -155,7 +155,8 @@ static void free_all(sph_ctx *c) {
   void *bufs[] = {c->x_can, c->v_can, c->xs_can, c->keys, c->keys_alt, c->ids, c->ids_alt,
                   c->hist, c->cell_count, c->cell_start, c->pos_a, c->pos_b, c->xold,
                   c->vel_a, c->vel_b, c->aux, c->dpbuf, c->rho, c->list, c->list_cnt,
-                  c->stats, c->pair_partial, c->red, c->sdf_buf, c->scan.tile_sums};
+                  c->stats, c->pair_partial, c->red, c->sdf_buf, c->scan.tile_sums,
+                  c->list16, c->tile_meta, c->tile_int};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto &p : c->prof.pending) {
@@ -266,12 +267,15 @@ extern "C" int sph_create(const sph_params *params, const sph_aabb *boundary, in
   };
   hipError_t e = hipGetDevice(&c->device);
   if (e != hipSuccess) return fail(hip_fail(e, "hipGetDevice"));
+  rc = configure_kernels();
+  if (rc) return fail(rc);
   e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) return fail(hip_fail(e, "hipStreamCreate"));
 
   const size_t cap = (size_t)std::max(max_particles, 1);
   const size_t f4 = sizeof(float4) * cap, u4 = sizeof(uint32_t) * cap;
-  const size_t groups = (cap + kWave - 1) / kWave;
+  const size_t tiles = (cap + kTile - 1) / kTile;
+  const size_t groups = tiles * (kTile / kWave);  // list groups cover whole tiles
   struct {
     void **p;
     size_t bytes;
@@ -286,9 +290,12 @@ extern "C" int sph_create(const sph_params *params, const sph_aabb *boundary, in
       {(void **)&c->vel_b, f4},       {(void **)&c->aux, f4},
       {(void **)&c->dpbuf, f4},       {(void **)&c->rho, sizeof(float) * cap},
       {(void **)&c->list, sizeof(uint32_t) * groups * kListStride},
+      {(void **)&c->list16, sizeof(uint16_t) * groups * kListStride},
       {(void **)&c->list_cnt, sizeof(int) * cap},
+      {(void **)&c->tile_meta, sizeof(int2) * tiles},
+      {(void **)&c->tile_int, sizeof(uint2) * tiles * kMaxInt},
       {(void **)&c->stats, sizeof(uint32_t) * STAT_WORDS},
-      {(void **)&c->pair_partial, sizeof(uint32_t) * ((cap + kBlock - 1) / kBlock)},
+      {(void **)&c->pair_partial, sizeof(uint32_t) * tiles},
       {(void **)&c->red, sizeof(double) * 4},
   };
   for (auto &a : allocs) {
@@ -602,6 +609,8 @@ extern "C" int sph_get_stats(sph_ctx *c, sph_stats *st) {
   st->cell_clamps =
       (unsigned long long)w[STAT_CLAMPS] | ((unsigned long long)w[STAT_CLAMPS + 1] << 32);
   st->substeps = c->substeps;
+  st->tiles = (int)c->ntiles;
+  st->tiles_global = (int)w[STAT_GLOBAL_TILES];
   if (c->pair_blocks) {
     std::vector<uint32_t> part;
     int rc = fetch(c, (const uint32_t *)c->pair_partial, c->pair_blocks, &part);
@@ -673,18 +682,37 @@ extern "C" int sph_download_neighbors(sph_ctx *c, uint32_t *out_ids, int *counts
   if (rc) return rc;
   if (!out_ids || !counts) return arg_error("NULL argument");
   const int n = c->n;
-  const size_t groups = ((size_t)n + kWave - 1) / kWave;
+  const size_t tiles = c->ntiles;
+  const size_t groups = tiles * (kTile / kWave);
   std::vector<uint32_t> ids, list;
+  std::vector<uint16_t> list16;
   std::vector<int> cnt;
+  std::vector<int2> meta;
+  std::vector<uint2> ints;
   if ((rc = fetch(c, c->id_s, (size_t)n, &ids))) return rc;
   if ((rc = fetch(c, (const int *)c->list_cnt, (size_t)n, &cnt))) return rc;
   if ((rc = fetch(c, (const uint32_t *)c->list, groups * kListStride, &list))) return rc;
+  if ((rc = fetch(c, (const uint16_t *)c->list16, groups * kListStride, &list16))) return rc;
+  if ((rc = fetch(c, (const int2 *)c->tile_meta, tiles, &meta))) return rc;
+  if ((rc = fetch(c, (const uint2 *)c->tile_int, tiles * kMaxInt, &ints))) return rc;
   for (int s = 0; s < n; ++s) {
     const uint32_t id = ids[s];
+    const size_t t = (size_t)s / kTile;
+    const int2 m = meta[t];
     counts[id] = cnt[s];
     const size_t base = (size_t)(s >> 6) * kListStride + (s & 63);
-    for (int k = 0; k < cnt[s]; ++k)
-      out_ids[(size_t)id * kMaxNb + k] = ids[list[base + (size_t)k * kWave]];
+    for (int k = 0; k < cnt[s]; ++k) {
+      uint32_t j;
+      if (m.x >= 0) {  // staged tile: LDS slot -> sorted index through the staging map
+        const uint32_t slot = list16[base + (size_t)k * kWave];
+        int i = m.x - 1;
+        while (i > 0 && ints[t * kMaxInt + i].y > slot) --i;
+        j = ints[t * kMaxInt + i].x + (slot - ints[t * kMaxInt + i].y);
+      } else {
+        j = list[base + (size_t)k * kWave];
+      }
+      out_ids[(size_t)id * kMaxNb + k] = ids[j];
+    }
   }
   return SPH_OK;
 }

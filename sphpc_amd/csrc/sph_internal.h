@@ -12,9 +12,19 @@
 namespace sph {
 
 constexpr int kMaxNb = SPH_MAX_NEIGHBORS;
-constexpr int kBlock = 256;          // threads per block for particle kernels
+constexpr int kBlock = 256;          // threads per block for elementwise kernels
 constexpr int kWave = 64;            // CDNA wavefront
-constexpr int kListStride = kWave * kMaxNb;  // list words per 64-particle group
+constexpr int kListStride = kWave * kMaxNb;  // list entries per 64-particle group
+
+// Neighbour tiles: kTile consecutive sorted particles form one workgroup of the
+// list build and of every gather pass.  The union of the tile's neighbour row
+// ranges (at most kMaxInt index intervals, at most kCap particles) is staged in
+// LDS and the tile's lists hold 16-bit LDS slots.  A tile that does not fit
+// keeps 32-bit global indices and gathers from global memory.
+constexpr int kTile = 128;
+constexpr int kCap = 2048;
+constexpr int kMaxInt = 32;
+constexpr int kMaxSeg = 16;          // merged intervals per (dz, dy) row in the build
 
 // Device-side constants of one run, all FP32 except the exact-predicate h^2.
 struct DevParams {
@@ -57,6 +67,7 @@ enum StatSlot {
   STAT_CLAMPS = 2,       // 64-bit counter (two words)
   STAT_ITERS = 4,
   STAT_CONVERGED = 5,
+  STAT_GLOBAL_TILES = 6, // tiles whose neighbourhood did not fit in LDS
   STAT_ERR0 = 8,         // per-iteration max error bits (kMaxIters + 1 slots)
   STAT_WORDS = 64
 };
@@ -115,8 +126,12 @@ struct sph_ctx {
   float4 *aux = nullptr;  // colour normal, then vorticity (w = |omega|)
   float4 *dpbuf = nullptr;  // staged delta p
   float *rho = nullptr;
-  uint32_t *list = nullptr;
+  uint32_t *list = nullptr;     // 32-bit sorted indices (tiles in global mode)
+  uint16_t *list16 = nullptr;   // 16-bit LDS slots (staged tiles)
   int *list_cnt = nullptr;
+  int2 *tile_meta = nullptr;    // (intervals, staged count); intervals < 0: global mode
+  uint2 *tile_int = nullptr;    // kMaxInt (start index, LDS offset) per tile
+  uint32_t ntiles = 0;
   uint32_t *stats = nullptr;  // sph::STAT_WORDS words
   uint32_t *pair_partial = nullptr;  // per-block list-length sums of the last search
   uint32_t pair_blocks = 0;
@@ -176,5 +191,6 @@ int launch_substep_init(sph_ctx *c);
 int launch_collide_soa(float *x, float *y, float *z, uint32_t n, const SdfDev &o,
                        hipStream_t s);
 int launch_kinetic_energy(sph_ctx *c, double *out_dev);
+int configure_kernels();  // LDS limits of the tiled kernels, once per process
 
 }  // namespace sph
