@@ -125,6 +125,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the timed substeps (for profiler runs)")
+    ap.add_argument("--net", default=os.path.join(ROOT, "sphpc_amd", "data", "warm_net_s0.008.bin"),
+                    help="trained warm-start network (sphpc_amd/warm_train.py)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run two 1 GiB device copies (counter calibration)")
     args = ap.parse_args()
@@ -199,18 +201,50 @@ def main():
                                      sorted(times.items(), key=lambda kv: -kv[1][0])}
     out["kernels_ms_per_substep"]["_sum"] = sum(v[0] for v in times.values()) / args.steps
     if not args.no_extras:
-        # Jacobi iterations to max density error <= 0.1% (cap 10), plain solver
-        ctx.set_params(gpu_params(sc, iters=10, tol=1e-3))
-        its, errs = [], []
-        for _ in range(args.tol_steps):
+        # Jacobi iterations and wall time to max density error <= 0.1% (cap 10),
+        # plain and warm-started, from the same state (the state after the
+        # timed substeps); the warm start needs a trained network file.
+        from sphpc_amd import warm_train
+        pos, vel = ctx.download()
+        tol_params = gpu_params(sc, iters=10, tol=1e-3)
+        net = None
+        if os.path.exists(args.net):
+            net = sa.Mlp()
+            net.load(args.net)
+        if net is not None:
+            cmp = warm_train.compare(ctx, net, pos, vel, tol_params, args.tol_steps,
+                                     log=lambda m: print(m, file=sys.stderr))
+        else:
+            ctx.upload(pos, vel)
+            ctx.set_params(tol_params)
+            cmp = {"plain": None, "warm": None}
+            its, errs, t_el = [], [], 0.0
             ctx.step(1)
-            s2 = ctx.stats()
-            its.append(s2.iterations)
-            errs.append(s2.max_density_error)
-        out["jacobi"] = {"tolerance": 1e-3, "cap": 10,
-                         "plain_iters_per_substep": float(np.mean(its)),
-                         "plain_max_density_error": float(np.max(errs)),
-                         "warm_iters_per_substep": None}
+            for _ in range(args.tol_steps):
+                t1 = time.perf_counter()
+                ctx.step(1)
+                ctx.synchronize()
+                t_el += time.perf_counter() - t1
+                s2 = ctx.stats()
+                its.append(s2.iterations)
+                errs.append(s2.max_density_error)
+            cmp["plain"] = dict(iters_per_substep=float(np.mean(its)),
+                                max_density_error=float(np.max(errs)),
+                                ms_per_substep=1e3 * t_el / args.tol_steps,
+                                particle_updates_per_s=n * args.tol_steps / t_el)
+        pl, wm = cmp["plain"], cmp["warm"]
+        out["jacobi"] = {"tolerance": 1e-3, "cap": 10, "substeps": args.tol_steps,
+                         "plain_iters_per_substep": pl["iters_per_substep"],
+                         "plain_max_density_error": pl["max_density_error"],
+                         "plain_particle_updates_per_s": pl["particle_updates_per_s"],
+                         "warm_iters_per_substep": wm["iters_per_substep"] if wm else None,
+                         "warm_max_density_error": wm["max_density_error"] if wm else None,
+                         "warm_particle_updates_per_s":
+                             wm["particle_updates_per_s"] if wm else None,
+                         "warm_speedup": (wm["particle_updates_per_s"] /
+                                          pl["particle_updates_per_s"]) if wm else None,
+                         "network": os.path.relpath(args.net, ROOT) if net else None}
+        ctx.upload(pos, vel)
         out["kinetic_energy_J"] = ctx.kinetic_energy()
     ctx.close()
     if not args.no_cpu and not args.no_extras:

@@ -156,7 +156,9 @@ enum sph_field {
   SPH_FIELD_VELOCITY = 4,      /* v in the velocity passes (3) */
   SPH_FIELD_COLOR = 5,         /* colour-field normal (3) */
   SPH_FIELD_VORTICITY = 6,     /* omega (3) */
-  SPH_FIELD_POSITION_OLD = 7   /* x at substep start (3) */
+  SPH_FIELD_POSITION_OLD = 7,  /* x at substep start (3) */
+  SPH_FIELD_LAMBDA_SUM = 8     /* accumulated multiplier of the last warm-started or
+                                  logged solve (1; see sph_set_warm_start) */
 };
 int sph_download_field(sph_ctx *ctx, int field, float *out);
 /* Neighbour lists as particle ids in the reference order (NeighborBucket,
@@ -264,6 +266,74 @@ int sph_profile_read(sph_ctx *ctx, int kernel, double *total_ms, long long *laun
 int sph_profile_reset(sph_ctx *ctx);
 const char *sph_profile_kernel_name(int kernel);
 int sph_profile_kernel_count(void);
+
+/* --- learned lambda warm start ------------------------------------------
+ * No counterpart in the reference: a 12 -> 64 -> 64 -> 1 SiLU network that
+ * predicts, per particle, the accumulated multiplier of the constraint solve
+ * (sum over iterations of stiffness(it) * lambda_it, particles.c:664,693-694)
+ * from quantities of the first density pass.  The plain Jacobi loop stays the
+ * acceptance test: with a tolerance, iterations continue until the density
+ * error is met, so the prediction can only save iterations.
+ *
+ * Dataset rows are SPH_MLP_STRIDE floats, device memory, 16-byte aligned:
+ *   0 neighbour count / 32           (NeighborBucket.count, neighbors.h:27)
+ *   1 rho / rho0 - 1                 (ComputeDensities, particles.c:512)
+ *   2 C_prev: -Lambda_prev * (|grad C|^2 + eps), previous substep
+ *   3 |grad C|^2 / (|grad C|^2 + eps) (sum_C, particles.c:654-661)
+ *   4 dt * div v (m/rho0-weighted spiky gradient of the predicted displacement)
+ *   5 SDF distance / h at x*, clamped to [-4, 4] (4 outside the grid)
+ *   6-8 normalised SDF gradient (0 outside the grid)
+ *   9-11 first density moment sum_j m W_ij (x_j - x_i) / (h rho_i)
+ *   12 |grad C|^2 + eps (not a network input; converts targets)
+ *   13-15 unused
+ * The target is C_eff = -Lambda * (|grad C|^2 + eps) (dimensionless); the
+ * solver applies Lambda_pred = -max(C_eff_pred, 0) / (|grad C|^2 + eps). */
+#define SPH_MLP_FEATURES 12
+#define SPH_MLP_STRIDE 16
+#define SPH_MLP_PARAMS 5313
+typedef struct sph_mlp sph_mlp;
+int sph_mlp_create(sph_mlp **out);  /* on the current device, own stream */
+int sph_mlp_destroy(sph_mlp *net);
+int sph_mlp_init(sph_mlp *net, unsigned long long seed);
+/* FP32 master parameters: W1[16][64], b1[64], W2[64][64], b2[64], w3[64], b3
+ * (row-major, row = input; W1 rows 12..15 are padding and stay zero). */
+int sph_mlp_set_weights(sph_mlp *net, const float *theta);
+int sph_mlp_get_weights(sph_mlp *net, float *theta);
+/* Inputs are normalised as (x - mean) / stddev (stddev 0: input ignored),
+ * the output is y * target_std + target_mean. */
+int sph_mlp_set_normalization(sph_mlp *net, const float *mean, const float *stddev,
+                              float target_mean, float target_std);
+int sph_mlp_get_normalization(sph_mlp *net, float *mean, float *stddev, float *target_mean,
+                              float *target_std);
+int sph_mlp_fit_normalization(sph_mlp *net, const float *x_dev, const float *t_dev,
+                              size_t count);
+/* Full-batch gradient of 0.5/B sum (y_n - t_n)^2 in normalised target units
+ * (host outputs; either may be NULL). */
+int sph_mlp_gradient(sph_mlp *net, const float *x_dev, const float *t_dev, size_t count,
+                     float *grad, float *loss);
+/* One Adam step (lr; beta 0.9/0.999, eps 1e-8) on `batch` rows drawn with
+ * replacement from `count` by a counter-based generator keyed on `seed`
+ * (batch 0: all rows in order).  loss != NULL waits for and returns the loss
+ * before the update. */
+int sph_mlp_train_step(sph_mlp *net, const float *x_dev, const float *t_dev, size_t count,
+                       size_t batch, float lr, unsigned long long seed, float *loss);
+int sph_mlp_reset_optimizer(sph_mlp *net);
+int sph_mlp_forward(sph_mlp *net, const float *x_dev, size_t count, float *y_dev);   /* FP32 */
+int sph_mlp_infer_bf16(sph_mlp *net, const float *x_dev, size_t count, float *y_dev); /* MFMA */
+int sph_mlp_synchronize(sph_mlp *net);
+int sph_mlp_save(sph_mlp *net, const char *path);
+int sph_mlp_load(sph_mlp *net, const char *path);
+
+/* Warm-start the solve of every following fused substep with `net` (its
+ * current weights are copied; NULL turns the warm start off). */
+int sph_set_warm_start(sph_ctx *ctx, sph_mlp *net);
+/* Record features and accumulated multipliers of every following fused
+ * substep (plain solve; takes precedence over the warm start). */
+int sph_set_sample_logging(sph_ctx *ctx, int enable);
+/* Copies every `every`-th particle's features and target of the last logged
+ * substep to device rows x_dev[offset..], t_dev[offset..]; *written = rows. */
+int sph_warm_samples(sph_ctx *ctx, float *x_dev, float *t_dev, size_t offset,
+                     size_t capacity, int every, size_t *written);
 
 #ifdef __cplusplus
 }

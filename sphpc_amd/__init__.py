@@ -22,7 +22,8 @@ FIELD_VELOCITY = 4
 FIELD_COLOR = 5
 FIELD_VORTICITY = 6
 FIELD_POSITION_OLD = 7
-_FIELD_COMPONENTS = {0: 3, 1: 1, 2: 1, 3: 3, 4: 3, 5: 3, 6: 3, 7: 3}
+FIELD_LAMBDA_SUM = 8
+_FIELD_COMPONENTS = {0: 3, 1: 1, 2: 1, 3: 3, 4: 3, 5: 3, 6: 3, 7: 3, 8: 1}
 
 
 class SphError(RuntimeError):
@@ -119,6 +120,26 @@ def lib():
     L.sph_profile_kernel_name.restype = C.c_char_p
     L.sph_profile_kernel_name.argtypes = [ip]
     L.sph_profile_kernel_count.argtypes = []
+    sz, ull = C.c_size_t, C.c_ulonglong
+    L.sph_mlp_create.argtypes = [C.POINTER(vp)]
+    L.sph_mlp_destroy.argtypes = [vp]
+    L.sph_mlp_init.argtypes = [vp, ull]
+    L.sph_mlp_set_weights.argtypes = [vp, fp]
+    L.sph_mlp_get_weights.argtypes = [vp, fp]
+    L.sph_mlp_set_normalization.argtypes = [vp, fp, fp, C.c_float, C.c_float]
+    L.sph_mlp_get_normalization.argtypes = [vp, fp, fp, fp, fp]
+    L.sph_mlp_fit_normalization.argtypes = [vp, vp, vp, sz]
+    L.sph_mlp_gradient.argtypes = [vp, vp, vp, sz, fp, fp]
+    L.sph_mlp_train_step.argtypes = [vp, vp, vp, sz, sz, C.c_float, ull, fp]
+    L.sph_mlp_reset_optimizer.argtypes = [vp]
+    L.sph_mlp_forward.argtypes = [vp, vp, sz, vp]
+    L.sph_mlp_infer_bf16.argtypes = [vp, vp, sz, vp]
+    L.sph_mlp_synchronize.argtypes = [vp]
+    L.sph_mlp_save.argtypes = [vp, C.c_char_p]
+    L.sph_mlp_load.argtypes = [vp, C.c_char_p]
+    L.sph_set_warm_start.argtypes = [vp, vp]
+    L.sph_set_sample_logging.argtypes = [vp, ip]
+    L.sph_warm_samples.argtypes = [vp, vp, vp, sz, sz, ip, C.POINTER(sz)]
     _lib = L
     return L
 
@@ -258,6 +279,20 @@ class Context:
         check(lib().sph_compute_energy(self._h, C.byref(e)))
         return e.value
 
+    def set_warm_start(self, net):
+        """Warm-start following fused substeps with `net` (an Mlp), None = off."""
+        check(lib().sph_set_warm_start(self._h, net.handle if net is not None else None))
+
+    def set_sample_logging(self, enable):
+        check(lib().sph_set_sample_logging(self._h, int(bool(enable))))
+
+    def warm_samples(self, x_buf, t_buf, offset, capacity, every=1):
+        """Appends the last logged substep's rows to device buffers; returns rows."""
+        w = C.c_size_t()
+        check(lib().sph_warm_samples(self._h, x_buf.ptr, t_buf.ptr, int(offset), int(capacity),
+                                     int(every), C.byref(w)))
+        return w.value
+
     # inspection
     def field(self, which):
         n = self.count()
@@ -300,6 +335,94 @@ class Context:
         ms, n = C.c_double(), C.c_longlong()
         check(lib().sph_profile_read(self._h, int(kernel), C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+
+MLP_FEATURES = 12
+MLP_STRIDE = 16
+MLP_PARAMS = 5313
+
+
+class Mlp:
+    """Lambda warm-start network (sph_mlp_*): FP32 training on the GPU, BF16 MFMA
+    inference.  Device arrays are DeviceBuffer objects (rows of MLP_STRIDE floats)."""
+
+    def __init__(self, seed=None):
+        self._h = C.c_void_p()
+        check(lib().sph_mlp_create(C.byref(self._h)))
+        if seed is not None:
+            check(lib().sph_mlp_init(self._h, int(seed)))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            lib().sph_mlp_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_weights(self, theta):
+        theta = _f32(theta)
+        assert theta.size == MLP_PARAMS
+        check(lib().sph_mlp_set_weights(self._h, _fptr(theta)))
+
+    def weights(self):
+        out = np.zeros(MLP_PARAMS, np.float32)
+        check(lib().sph_mlp_get_weights(self._h, _fptr(out)))
+        return out
+
+    def set_normalization(self, mean, std, target_mean, target_std):
+        m, s = _f32(mean), _f32(std)
+        check(lib().sph_mlp_set_normalization(self._h, _fptr(m), _fptr(s), float(target_mean),
+                                              float(target_std)))
+
+    def normalization(self):
+        m, s = np.zeros(MLP_FEATURES, np.float32), np.zeros(MLP_FEATURES, np.float32)
+        tm, ts = C.c_float(), C.c_float()
+        check(lib().sph_mlp_get_normalization(self._h, _fptr(m), _fptr(s), C.byref(tm),
+                                              C.byref(ts)))
+        return m, s, tm.value, ts.value
+
+    def fit_normalization(self, x_buf, t_buf, count):
+        check(lib().sph_mlp_fit_normalization(self._h, x_buf.ptr, t_buf.ptr, int(count)))
+
+    def gradient(self, x_buf, t_buf, count):
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = C.c_float()
+        check(lib().sph_mlp_gradient(self._h, x_buf.ptr, t_buf.ptr, int(count), _fptr(g),
+                                     C.byref(loss)))
+        return loss.value, g
+
+    def train_step(self, x_buf, t_buf, count, batch, lr, seed, want_loss=False):
+        loss = C.c_float()
+        check(lib().sph_mlp_train_step(self._h, x_buf.ptr, t_buf.ptr, int(count), int(batch),
+                                       float(lr), int(seed),
+                                       C.byref(loss) if want_loss else None))
+        return loss.value if want_loss else None
+
+    def reset_optimizer(self):
+        check(lib().sph_mlp_reset_optimizer(self._h))
+
+    def forward(self, x_buf, count, y_buf):
+        check(lib().sph_mlp_forward(self._h, x_buf.ptr, int(count), y_buf.ptr))
+
+    def infer_bf16(self, x_buf, count, y_buf):
+        check(lib().sph_mlp_infer_bf16(self._h, x_buf.ptr, int(count), y_buf.ptr))
+
+    def synchronize(self):
+        check(lib().sph_mlp_synchronize(self._h))
+
+    def save(self, path):
+        check(lib().sph_mlp_save(self._h, str(path).encode()))
+
+    def load(self, path):
+        check(lib().sph_mlp_load(self._h, str(path).encode()))
 
 
 def kernel_names():

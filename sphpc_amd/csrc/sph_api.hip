@@ -81,7 +81,7 @@ static const char *kKernelNames[K_COUNT] = {
     "predict_hash", "hash", "gravity", "predict", "radix_upsweep", "radix_downsweep",
     "scan", "reorder", "build_lists", "density_lambda", "density", "lambda", "dp",
     "dp_update", "position_stars", "velocity", "color", "surface", "xsph", "vorticity",
-    "confine", "commit", "aux"};
+    "confine", "commit", "aux", "warm_lambda"};
 
 // Derived FP32 constants (SetParameters, simulation.c:147-177; the constants
 // of particles.c named in DevParams).  Computed in double, rounded once.
@@ -126,6 +126,7 @@ static void derive(sph_ctx *c) {
   d.conf_k = (float)(P.vorticity_model == 1 ? P.time_step * P.vorticity_epsilon
                                             : P.time_step * P.vorticity_epsilon / m);
   d.color_k = (float)(h * m * DelW_poly_norm);
+  d.div_k = (float)(m / rho0 * DelW_norm);
   const double lo[3] = {c->box.min_x, c->box.min_y, c->box.min_z};
   const double hi[3] = {c->box.max_x, c->box.max_y, c->box.max_z};
   for (int a = 0; a < 3; ++a) {
@@ -145,7 +146,7 @@ static int check_params(const sph_params *p) {
   if (!(p->rest_density > 0.0) || !(p->rest_mass > 0.0))
     return arg_error("rest_density and rest_mass must be positive");
   if (p->solve_iterations < 0 || p->solve_iterations > kMaxIters)
-    return arg_error("solve_iterations out of range [0, 32]");
+    return arg_error("solve_iterations out of range [0, 128]");
   if (p->vorticity_model != 0 && p->vorticity_model != 1)
     return arg_error("vorticity_model must be 0 or 1");
   return SPH_OK;
@@ -156,7 +157,8 @@ static void free_all(sph_ctx *c) {
                   c->hist, c->cell_count, c->cell_start, c->pos_a, c->pos_b, c->xold,
                   c->vel_a, c->vel_b, c->aux, c->dpbuf, c->rho, c->list, c->list_cnt,
                   c->stats, c->pair_partial, c->red, c->sdf_buf, c->scan.tile_sums,
-                  c->list16, c->tile_meta, c->tile_int};
+                  c->list16, c->tile_meta, c->tile_int, c->lam_acc, c->lam_prev, c->feat,
+                  c->warm_net};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto &p : c->prof.pending) {
@@ -412,11 +414,13 @@ extern "C" int sph_upload_state(sph_ctx *c, int n, const float *x, const float *
     SPH_HIP(hipMemcpyAsync(c->v_can, hv.data(), sizeof(float4) * n, hipMemcpyHostToDevice,
                            c->stream));
   }
+  if (c->lam_prev) SPH_HIP(hipMemsetAsync(c->lam_prev, 0, sizeof(float) * c->cap, c->stream));
   SPH_HIP(hipStreamSynchronize(c->stream));
   c->n = n;
   c->state_valid = true;
   c->sorted_valid = false;
   c->lists_valid = false;
+  c->feat_substep = -1;
   return SPH_OK;
 }
 
@@ -544,9 +548,19 @@ static int substep_fused(sph_ctx *c) {
   rc = launch_neighbor_search(c, true);
   if (rc) return rc;
   const int iters = c->P.solve_iterations;
-  for (int it = 0; it < iters; ++it) {
+  // Logged or warm-started solve: the first iteration's lambda pass also
+  // computes the network features (and, warm, replaces lambda by the
+  // network's accumulated multiplier); later iterations are the plain ones.
+  const bool warm = iters > 0 && (c->log_on || c->warm_on);
+  int it0 = 0;
+  if (warm) {
+    if ((rc = launch_warm_lambda(c, c->log_on))) return rc;
+    if ((rc = launch_dp_update(c, 0, 0, true))) return rc;
+    it0 = 1;
+  }
+  for (int it = it0; it < iters; ++it) {
     if ((rc = launch_density(c, 0, it))) return rc;
-    if ((rc = launch_dp_update(c, it, it))) return rc;
+    if ((rc = launch_dp_update(c, it, it, warm))) return rc;
   }
   c->last_err_slot = iters > 0 ? iters - 1 : -1;
   if (c->P.density_tolerance > 0.0) {
@@ -567,7 +581,67 @@ static int substep_fused(sph_ctx *c) {
   } else {
     if ((rc = launch_commit(c))) return rc;
   }
+  if (warm && (rc = launch_scatter_lambda(c))) return rc;
   c->substeps++;
+  if (warm && c->log_on) c->feat_substep = c->substeps;
+  return SPH_OK;
+}
+
+// Buffers of the warm start / logging path, allocated on first use.
+static int ensure_warm_buffers(sph_ctx *c, bool features) {
+  const size_t cap = (size_t)std::max(c->cap, 1);
+  if (!c->lam_acc) SPH_HIP(hipMalloc((void **)&c->lam_acc, sizeof(float) * cap));
+  if (!c->lam_prev) {
+    SPH_HIP(hipMalloc((void **)&c->lam_prev, sizeof(float) * cap));
+    SPH_HIP(hipMemsetAsync(c->lam_prev, 0, sizeof(float) * cap, c->stream));
+  }
+  if (features && !c->feat)
+    SPH_HIP(hipMalloc((void **)&c->feat, sizeof(float) * kMlpPad * cap));
+  SPH_HIP(hipStreamSynchronize(c->stream));
+  return SPH_OK;
+}
+
+extern "C" int sph_set_warm_start(sph_ctx *c, sph_mlp *net) {
+  if (!c) return arg_error("context is NULL");
+  if (!net) {
+    c->warm_on = false;
+    return SPH_OK;
+  }
+  int rc = ensure_warm_buffers(c, false);
+  if (rc) return rc;
+  if (!c->warm_net) SPH_HIP(hipMalloc((void **)&c->warm_net, sizeof(MlpPacked)));
+  rc = mlp_pack(net, c->warm_net, c->stream);
+  if (rc) return rc;
+  SPH_HIP(hipStreamSynchronize(c->stream));
+  c->warm_on = true;
+  return SPH_OK;
+}
+
+extern "C" int sph_set_sample_logging(sph_ctx *c, int enable) {
+  if (!c) return arg_error("context is NULL");
+  if (enable) {
+    int rc = ensure_warm_buffers(c, true);
+    if (rc) return rc;
+  }
+  c->log_on = enable != 0;
+  return SPH_OK;
+}
+
+extern "C" int sph_warm_samples(sph_ctx *c, float *x, float *t, size_t offset, size_t capacity,
+                                int every, size_t *written) {
+  if (!c || !x || !t) return arg_error("NULL argument");
+  if (every < 1) return arg_error("every must be >= 1");
+  if (written) *written = 0;
+  if (c->feat_substep < 0 || c->feat_substep != c->substeps)
+    return state_error("the last substep was not logged (sph_set_sample_logging)");
+  if (offset > capacity) return arg_error("offset beyond capacity");
+  if ((((uintptr_t)x) & 15u) != 0) return arg_error("feature rows must be 16-byte aligned");
+  size_t count = ((size_t)c->n + (size_t)every - 1) / (size_t)every;
+  count = std::min(count, capacity - offset);
+  int rc = launch_warm_samples(c, x + offset * kMlpPad, t + offset, (uint32_t)count, every);
+  if (rc) return rc;
+  SPH_HIP(hipStreamSynchronize(c->stream));
+  if (written) *written = count;
   return SPH_OK;
 }
 
@@ -658,6 +732,12 @@ extern "C" int sph_download_field(sph_ctx *c, int field, float *out) {
       comps = 1;
       rc = fetch(c, (const float4 *)c->pos, n, &v4);
       break;
+    case SPH_FIELD_LAMBDA_SUM:
+      // already in particle-id order
+      if (!c->lam_prev) return state_error("no warm start or logging enabled");
+      if ((rc = fetch(c, (const float *)c->lam_prev, n, &v1))) return rc;
+      memcpy(out, v1.data(), sizeof(float) * n);
+      return SPH_OK;
     default: return arg_error("unknown field");
   }
   if (rc) return rc;

@@ -47,6 +47,7 @@ struct DevParams {
   float vort_eps;
   float conf_k;              // dt * vort_eps / m  (model 0) or dt * vort_eps (model 1)
   float color_k;             // h * m * DelW_poly_norm
+  float div_k;               // m / rho0 * DelW_norm (velocity divergence feature)
   float box_lo[3], box_hi[3], box_hi_clamp[3];  // clamp to [lo, hi - 1e-4 h)
   uint32_t nx, ny, nz, nxy;
   int vort_model;
@@ -69,15 +70,36 @@ enum StatSlot {
   STAT_CONVERGED = 5,
   STAT_GLOBAL_TILES = 6, // tiles whose neighbourhood did not fit in LDS
   STAT_ERR0 = 8,         // per-iteration max error bits (kMaxIters + 1 slots)
-  STAT_WORDS = 64
+  STAT_WORDS = 256
 };
-constexpr int kMaxIters = 32;
+constexpr int kMaxIters = 128;
+
+// Lambda warm-start network: 12 features (padded to 16) -> 64 SiLU -> 64 SiLU -> 1.
+// FP32 master parameters in one vector: W1[16][64] (row = input), b1[64],
+// W2[64][64] (row = input), b2[64], w3[64], b3.
+constexpr int kMlpIn = 12, kMlpPad = 16, kMlpH = 64;
+constexpr int kMlpW1 = 0, kMlpB1 = kMlpW1 + kMlpPad * kMlpH, kMlpW2 = kMlpB1 + kMlpH,
+              kMlpB2 = kMlpW2 + kMlpH * kMlpH, kMlpW3 = kMlpB2 + kMlpH,
+              kMlpB3 = kMlpW3 + kMlpH, kMlpParams = kMlpB3 + 1;  // 5313
+// Normalisation words: feature means, reciprocal deviations, target shift/scale.
+constexpr int kNormMu = 0, kNormIsd = 16, kNormYShift = 32, kNormYScale = 33, kNormWords = 36;
+
+// BF16 inference weights packed in MFMA operand order (layout in sph_mlp_dev.h).
+struct MlpPacked {
+  uint4 a1[2 * 64];        // layer 1 A operands [row block][lane], 8 bf16 each
+  uint4 a2[2 * 4 * 64];    // layer 2 A operands [row block][k chunk][lane]
+  float4 b1[2 * 2 * 4];    // layer 1 bias [row block][lane half][accumulator / 4]
+  float4 b2[2 * 2 * 4];    // layer 2 bias, same order
+  float4 w3[2 * 2 * 4];    // output weights (bf16 values), same order
+  float norm[kNormWords];
+  float b3;
+};
 
 enum KernelId {
   K_PREDICT_HASH = 0, K_HASH, K_GRAVITY, K_PREDICT, K_RADIX_UP, K_RADIX_DOWN, K_SCAN,
   K_REORDER, K_BUILD_LISTS, K_DENSITY_LAMBDA, K_DENSITY, K_LAMBDA, K_DP, K_DP_UPDATE,
   K_POS_STARS, K_VELOCITY, K_COLOR, K_SURFACE, K_XSPH, K_VORTICITY, K_CONFINE, K_COMMIT,
-  K_AUX, K_COUNT
+  K_AUX, K_WARM_LAMBDA, K_COUNT
 };
 
 struct ScanScratch {
@@ -136,6 +158,14 @@ struct sph_ctx {
   uint32_t *pair_partial = nullptr;  // per-block list-length sums of the last search
   uint32_t pair_blocks = 0;
   double *red = nullptr;      // FP64 reduction scratch
+  // learned warm start
+  float *lam_acc = nullptr;   // sorted: sum over iterations of stiffness * lambda
+  float *lam_prev = nullptr;  // canonical: lam_acc of the previous substep
+  float *feat = nullptr;      // sorted feature rows (kMlpPad floats) when logging
+  sph::MlpPacked *warm_net = nullptr;
+  bool warm_on = false;
+  bool log_on = false;
+  long long feat_substep = -1;  // substep count after the last logged substep
   int last_err_slot = -1;
   int staged_slot = 0;
   bool state_valid = false;
@@ -179,7 +209,7 @@ int launch_neighbor_search(sph_ctx *c, bool fused);
 int launch_density(sph_ctx *c, int mode, int err_slot);
 int launch_dp(sph_ctx *c, int substep);
 int launch_position_stars(sph_ctx *c);
-int launch_dp_update(sph_ctx *c, int substep, int err_slot);
+int launch_dp_update(sph_ctx *c, int substep, int err_slot, bool accumulate = false);
 int launch_velocity(sph_ctx *c);
 int launch_color(sph_ctx *c, bool with_velocity);
 int launch_surface(sph_ctx *c);
@@ -191,6 +221,10 @@ int launch_substep_init(sph_ctx *c);
 int launch_collide_soa(float *x, float *y, float *z, uint32_t n, const SdfDev &o,
                        hipStream_t s);
 int launch_kinetic_energy(sph_ctx *c, double *out_dev);
+int launch_warm_lambda(sph_ctx *c, bool log);
+int launch_scatter_lambda(sph_ctx *c);
+int launch_warm_samples(sph_ctx *c, float *x, float *t, uint32_t count, int every);
+int mlp_pack(sph_mlp *m, MlpPacked *dst, hipStream_t s);  // sph_mlp.hip
 int configure_kernels();  // LDS limits of the tiled kernels, once per process
 
 }  // namespace sph

@@ -20,6 +20,7 @@
 // Lists are slot-major per 64-particle group: entry k of particle s is at
 // (s/64)*64*60 + k*64 + s%64, so a wave reading entry k reads contiguously.
 #include "sph_internal.h"
+#include "sph_mlp_dev.h"
 
 namespace sph {
 
@@ -713,13 +714,15 @@ __global__ __launch_bounds__(kBlock) void k_position_stars(uint32_t n, float4 *_
 
 // Fused UpdateDPs + UpdatePositionStars: reads pin (x*, lambda), writes pout
 // (x*', rho).  In tolerance mode, once the density pass of this iteration is
-// within tolerance, the particles are copied unchanged instead.
+// within tolerance, the particles are copied unchanged instead.  With lam_acc,
+// adds stiff * lambda_i to the particle's accumulated multiplier.
 __global__ __launch_bounds__(kTile) void k_dp_update(uint32_t n, TileLists TL,
                                                      const float4 *__restrict__ pin,
                                                      float4 *__restrict__ pout,
                                                      const float *__restrict__ rho, DevParams P,
                                                      float scale, SdfDev o, uint32_t *stats,
-                                                     int err_slot, float tol) {
+                                                     int err_slot, float tol,
+                                                     float *__restrict__ lam_acc, float stiff) {
   extern __shared__ float4 lds4[];
   bool done = false;
   if (tol > 0.0f) {
@@ -746,6 +749,153 @@ __global__ __launch_bounds__(kTile) void k_dp_update(uint32_t n, TileLists TL,
   apply_boundary(x, y, z, P);
   collide(x, y, z, o);
   pout[s] = make_float4(x, y, z, rho[s]);
+  if (lam_acc) lam_acc[s] += stiff * p.w;
+}
+
+// ---------------------------------------------------------------------------
+// Learned warm start
+// ---------------------------------------------------------------------------
+// SDF distance (in h, clamped to [-4, 4]) and unit gradient at a point, with
+// the sampling of the collision pass (obstacle_cuda.cu:33-53,88).
+__device__ __forceinline__ void sdf_features(float x, float y, float z, const SdfDev &o,
+                                             const DevParams &P, float *f) {
+  f[0] = 4.0f;
+  f[1] = f[2] = f[3] = 0.0f;
+  if (!o.data) return;
+  if (!(x > o.wmin[0] && x < o.wmax[0] && y > o.wmin[1] && y < o.wmax[1] && z > o.wmin[2] &&
+        z < o.wmax[2]))
+    return;
+  const float u = (x - o.wmin[0]) / o.len[0];
+  const float v = (y - o.wmin[1]) / o.len[1];
+  const float w = (z - o.wmin[2]) / o.len[2];
+  const float d = sdf_sample(o, u, v, w);
+  f[0] = fminf(fmaxf(d * o.m2w * P.inv_h, -4.0f), 4.0f);
+  const float gx = sdf_sample(o, u + o.inv_dim[0], v, w) - sdf_sample(o, u - o.inv_dim[0], v, w);
+  const float gy = sdf_sample(o, u, v + o.inv_dim[1], w) - sdf_sample(o, u, v - o.inv_dim[1], w);
+  const float gz = sdf_sample(o, u, v, w + o.inv_dim[2]) - sdf_sample(o, u, v, w - o.inv_dim[2]);
+  const float dd = gx * gx + gy * gy + gz * gz;
+  if (!(dd > 0.0f)) return;
+  const float inv = 1.0f / sqrtf(dd);
+  f[1] = gx * inv;
+  f[2] = gy * inv;
+  f[3] = gz * inv;
+}
+
+// First pass of a warm-started or logged solve.  ComputeDensities and
+// ComputeLambda as in k_density_lambda, plus the network features, in one
+// gather over LDS-staged (x*, x) pairs.  LOG: writes the feature rows and the
+// plain lambda.  Otherwise runs the network on MFMA (BF16, FP32 accumulate)
+// and writes Lambda_pred = -max(C_eff, 0) / (|grad C|^2 + eps) as lambda.
+// Either way the particle's accumulated multiplier restarts at zero.
+template <bool LOG>
+__global__ __launch_bounds__(kTile) void k_warm_lambda(
+    uint32_t n, TileLists TL, float4 *pos, const float4 *__restrict__ xold, DevParams P,
+    float *__restrict__ rho_io, uint32_t *stats, const uint32_t *__restrict__ id_s,
+    const float *__restrict__ lam_prev, SdfDev o, const MlpPacked *__restrict__ net,
+    float *__restrict__ lam_acc, float *__restrict__ feat) {
+  extern __shared__ Pair4 lds8[];
+  const uint32_t t = xcd_block(blockIdx.x, TL.ntiles);
+  const uint32_t s = t * kTile + threadIdx.x;
+  auto gload = [&](uint32_t j) { return Pair4{pos[j], xold[j]}; };
+  const bool staged = stage_tile(t, TL, lds8, gload);
+  const bool active = s < n;
+  float f[kMlpPad];
+#pragma unroll
+  for (int i = 0; i < kMlpPad; ++i) f[i] = 0.0f;
+  float err = 0.0f, lam0 = 0.0f, denom = 1.0f;
+  if (active) {
+    const float4 p = pos[s];
+    const float4 po = xold[s];
+    const float ux = p.x - po.x, uy = p.y - po.y, uz = p.z - po.z;  // dt * v_i
+    const int c = TL.cnt[s];
+    float su3 = 0.0f, sgx = 0.0f, sgy = 0.0f, sgz = 0.0f, sc = 0.0f;
+    float mx = 0.0f, my = 0.0f, mz = 0.0f, dv = 0.0f;
+    visit(staged, TL, s, c, lds8, gload, [&](const Pair4 &qq) {
+      const float4 &q = qq.a, &qo = qq.b;
+      const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      const float u = P.h2 - r2;
+      const float w3 = u > 0.0f ? u * u * u : 0.0f;
+      su3 += w3;
+      mx -= w3 * dx;
+      my -= w3 * dy;
+      mz -= w3 * dz;
+      const float r2c = fmaxf(r2, P.rmin2);
+      const float ir = fast_rsq(r2c);
+      const float hr = P.h - r2c * ir;
+      const float g = hr > 0.0f ? hr * hr * ir : 0.0f;
+      sgx += g * dx;
+      sgy += g * dy;
+      sgz += g * dz;
+      sc += g * g * r2;
+      const float ex = (q.x - qo.x) - ux, ey = (q.y - qo.y) - uy, ez = (q.z - qo.z) - uz;
+      dv += g * (ex * dx + ey * dy + ez * dz);
+    });
+    const float rho = P.mW0 + P.mWn * su3;
+    rho_io[s] = rho;
+    const float cr = (rho - P.rest_density) * P.inv_rest_density;
+    err = fmaxf(cr, 0.0f);
+    const float gc2 = P.lam_k2 * (sc + sgx * sgx + sgy * sgy + sgz * sgz);
+    denom = gc2 + P.lambda_eps;
+    lam0 = -err / denom;
+    f[0] = (float)c * (1.0f / 32.0f);
+    f[1] = cr;
+    f[2] = -lam_prev[id_s[s]] * denom;
+    f[3] = gc2 / denom;
+    f[4] = P.div_k * dv;
+    sdf_features(p.x, p.y, p.z, o, P, f + 5);
+    const float mk = P.mWn / (rho * P.h);
+    f[9] = mk * mx;
+    f[10] = mk * my;
+    f[11] = mk * mz;
+  }
+  wave_max_to(&stats[STAT_ERR0], err, active);
+  float lam = lam0;
+  if (LOG) {
+    if (active) {
+      float4 *row = reinterpret_cast<float4 *>(feat + (size_t)s * kMlpPad);
+      row[0] = make_float4(f[0], f[1], f[2], f[3]);
+      row[1] = make_float4(f[4], f[5], f[6], f[7]);
+      row[2] = make_float4(f[8], f[9], f[10], f[11]);
+      row[3] = make_float4(denom, lam0, 0.0f, 0.0f);
+    }
+  } else {
+    const float y = mlp_wave(f, net);  // every lane of the wave takes part
+    lam = -fmaxf(y, 0.0f) / denom;
+  }
+  if (active) {
+    pos[s].w = lam;
+    lam_acc[s] = 0.0f;
+  }
+}
+
+// Keeps each particle's accumulated multiplier in particle-id order.
+__global__ __launch_bounds__(kBlock) void k_scatter_lambda(uint32_t n,
+                                                           const uint32_t *__restrict__ id_s,
+                                                           const float *__restrict__ lam_acc,
+                                                           float *__restrict__ lam_prev) {
+  const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+  if (s < n) lam_prev[id_s[s]] = lam_acc[s];
+}
+
+// Dataset rows of a logged substep: every `every`-th sorted particle's
+// feature row and its target C_eff = -Lambda * (|grad C|^2 + eps).
+__global__ __launch_bounds__(kBlock) void k_warm_samples(uint32_t count, uint32_t every,
+                                                         const float *__restrict__ feat,
+                                                         const float *__restrict__ lam_acc,
+                                                         float *__restrict__ x,
+                                                         float *__restrict__ t) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= count) return;
+  const size_t s = (size_t)k * every;
+  const float4 *src = reinterpret_cast<const float4 *>(feat + s * kMlpPad);
+  float4 *dst = reinterpret_cast<float4 *>(x + (size_t)k * kMlpPad);
+  const float4 r3 = src[3];
+  dst[0] = src[0];
+  dst[1] = src[1];
+  dst[2] = src[2];
+  dst[3] = r3;
+  t[k] = -lam_acc[s] * r3.x;
 }
 
 // ---------------------------------------------------------------------------
@@ -1041,7 +1191,8 @@ int configure_kernels() {
   static int done = 0;
   if (done) return SPH_OK;
   const void *big[] = {(const void *)k_surface, (const void *)k_xsph,
-                       (const void *)k_vorticity};
+                       (const void *)k_vorticity, (const void *)k_warm_lambda<true>,
+                       (const void *)k_warm_lambda<false>};
   for (const void *f : big)
     SPH_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLds8));
   done = 1;
@@ -1173,16 +1324,57 @@ int launch_position_stars(sph_ctx *c) {
   return SPH_OK;
 }
 
-int launch_dp_update(sph_ctx *c, int substep, int err_slot) {
+static float stiffness_unitless(const sph_ctx *c, int substep) {
+  return (float)(1.0 - pow(1.0 - c->P.stiffness_k, 1.0 / (double)substep));
+}
+
+int launch_dp_update(sph_ctx *c, int substep, int err_slot, bool accumulate) {
   const uint32_t n = c->n;
   if (!n) return SPH_OK;
   float4 *out = (c->pos == c->pos_a) ? c->pos_b : c->pos_a;
   LAUNCH_TIMED(c, K_DP_UPDATE,
                k_dp_update<<<c->ntiles, kTile, kLds4, c->stream>>>(
                    n, tile_lists(c), c->pos, out, c->rho, c->dev, stiffness_scale(c, substep),
-                   c->sdf, c->stats, err_slot, (float)c->P.density_tolerance));
+                   c->sdf, c->stats, err_slot, (float)c->P.density_tolerance,
+                   accumulate ? c->lam_acc : nullptr, stiffness_unitless(c, substep)));
   SPH_HIP(hipGetLastError());
   c->pos = out;
+  return SPH_OK;
+}
+
+int launch_warm_lambda(sph_ctx *c, bool log) {
+  const uint32_t n = c->n;
+  if (!n) return SPH_OK;
+  const TileLists TL = tile_lists(c);
+  if (log)
+    LAUNCH_TIMED(c, K_WARM_LAMBDA,
+                 k_warm_lambda<true><<<c->ntiles, kTile, kLds8, c->stream>>>(
+                     n, TL, c->pos, c->xold, c->dev, c->rho, c->stats, c->id_s, c->lam_prev,
+                     c->sdf, nullptr, c->lam_acc, c->feat));
+  else
+    LAUNCH_TIMED(c, K_WARM_LAMBDA,
+                 k_warm_lambda<false><<<c->ntiles, kTile, kLds8, c->stream>>>(
+                     n, TL, c->pos, c->xold, c->dev, c->rho, c->stats, c->id_s, c->lam_prev,
+                     c->sdf, c->warm_net, c->lam_acc, nullptr));
+  SPH_HIP(hipGetLastError());
+  return SPH_OK;
+}
+
+int launch_scatter_lambda(sph_ctx *c) {
+  const uint32_t n = c->n;
+  if (!n) return SPH_OK;
+  LAUNCH_TIMED(c, K_AUX,
+               k_scatter_lambda<<<blocks_for(n), kBlock, 0, c->stream>>>(n, c->id_s, c->lam_acc,
+                                                                         c->lam_prev));
+  SPH_HIP(hipGetLastError());
+  return SPH_OK;
+}
+
+int launch_warm_samples(sph_ctx *c, float *x, float *t, uint32_t count, int every) {
+  if (!count) return SPH_OK;
+  k_warm_samples<<<blocks_for(count), kBlock, 0, c->stream>>>(count, (uint32_t)every, c->feat,
+                                                              c->lam_acc, x, t);
+  SPH_HIP(hipGetLastError());
   return SPH_OK;
 }
 

@@ -1,0 +1,215 @@
+"""Trains the lambda warm-start network on the GPU from the solver's own samples.
+
+    python -m sphpc_amd.warm_train --out sphpc_amd/data/warm_net_s0.008.bin
+
+Scenes (seeded): one fluid block per scene with sides drawn from U(0.2, 1.0) m
+on the s = 0.008 m lattice (jitter 0.01 s), at a random position in the
+6 x 2 x 2 m box with the torus + box obstacle and the physics of the 8M
+double-dam-break workload.  Each scene runs `--substeps` substeps in the
+tolerance mode the warm start is used in (density error 1e-3, cap 10).  Every
+`--log-every` substeps, two consecutive substeps are solved with logging to
+1e-4 (cap 100): the first gives the previous-multiplier feature, the second
+is sampled.  Samples are (12 features, C_eff) rows, C_eff = -Lambda (|grad C|^2
++ eps) with Lambda the accumulated multiplier of the converged plain solve.
+
+Training: Adam on random mini-batches with the project's HIP forward/backward
+kernels, cosine learning-rate decay; the last scenes are held out for the
+validation loss.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+import sphpc_amd as sa  # noqa: E402
+
+SPACING = float(np.float32(0.008))
+BOX = (6.0, 2.0, 2.0)
+
+
+def workload_params(iters=10, tol=1e-3):
+    s = SPACING
+    return sa.default_params(2.0 * s, time_step=1.0 / 300.0, g=9.81, k=0.5, c=0.01,
+                             rest_density=1000.0, rest_mass=1000.0 * s ** 3,
+                             lambda_epsilon=600.0, vorticity_epsilon=0.01, vorticity_model=1,
+                             solve_iterations=iters, density_tolerance=tol)
+
+
+def workload_obstacle(dims=(256, 256, 256)):
+    """Torus (R 0.4, r 0.12, axis z) united with a box, sampled onto a grid."""
+    wmin, wmax = (2.2, 0.2, 0.0), (3.8, 1.8, 1.6)
+    shapes = [sa.torus((3.0, 0.75, 0.12), 0.4, 0.12, axis=2),
+              sa.box_shape((3.0, 1.45, 0.25), (0.25, 0.2, 0.25))]
+    sdf = sa.sdf_sample_shapes(shapes, dims, wmin, wmax)
+    return dict(sdf=sdf, dims=dims, wmin=wmin, wmax=wmax, dx=(wmax[0] - wmin[0]) / dims[0])
+
+
+def random_block(rng):
+    sides = rng.uniform(0.2, 1.0, 3)
+    counts = [max(1, int(a / SPACING)) for a in sides]
+    ext = [c * SPACING for c in counts]
+    origin = [rng.uniform(0.0, BOX[i] - ext[i]) for i in range(3)]
+    return sa.Block(*counts, *origin)
+
+
+def generate(ctx, xb, tb, capacity, scenes, seed, substeps, log_every, every, log=print):
+    """Runs the scenes and appends samples to the device buffers.  Returns the
+    number of rows and the row offset at which each scene starts."""
+    rows = 0
+    starts = []
+    plain = workload_params(10, 1e-3)
+    logged = workload_params(100, 1e-4)
+    for sc in range(scenes):
+        rng = np.random.default_rng(seed * 1000 + sc)
+        pos = sa.scene_lattice([random_block(rng)], SPACING, jitter=0.01, seed=seed * 1000 + sc)
+        ctx.upload(pos)
+        starts.append(rows)
+        t0 = time.perf_counter()
+        iters = []
+        for k in range(1, substeps + 1):
+            sample = k % log_every == 0
+            pre = (k + 1) % log_every == 0
+            if sample or pre:
+                ctx.set_params(logged)
+                ctx.set_sample_logging(True)
+            ctx.step(1)
+            if sample:
+                st = ctx.stats()
+                iters.append(st.iterations)
+                rows += ctx.warm_samples(xb, tb, rows, capacity, every)
+            if sample or pre:
+                ctx.set_sample_logging(False)
+                ctx.set_params(plain)
+        log(f"[warm_train] scene {sc}: {len(pos)} particles, rows {rows}, "
+            f"logged solve iterations {iters}, {time.perf_counter() - t0:.1f} s")
+        if rows >= capacity:
+            break
+    return rows, starts
+
+
+def train(net, xb, tb, count, steps, batch, lr, seed, log=print):
+    """Adam with cosine decay from lr to lr/20 on rows [0, count)."""
+    net.reset_optimizer()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        a = lr * (0.05 + 0.95 * 0.5 * (1.0 + math.cos(math.pi * k / max(steps - 1, 1))))
+        want = (k % 500 == 0) or k == steps - 1
+        loss = net.train_step(xb, tb, count, batch, a, seed * 1_000_003 + k, want_loss=want)
+        if want:
+            log(f"[warm_train] step {k}: batch loss {loss:.4g} (lr {a:.3g}, "
+                f"{time.perf_counter() - t0:.1f} s)")
+    net.synchronize()
+    return time.perf_counter() - t0
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--out", default=os.path.join(ROOT, "sphpc_amd", "data",
+                                                  "warm_net_s0.008.bin"))
+    ap.add_argument("--scenes", type=int, default=64)
+    ap.add_argument("--holdout", type=int, default=8)
+    ap.add_argument("--substeps", type=int, default=60)
+    ap.add_argument("--log-every", type=int, default=10)
+    ap.add_argument("--every", type=int, default=1, help="sample every k-th particle")
+    ap.add_argument("--max-rows", type=int, default=60_000_000)
+    ap.add_argument("--steps", type=int, default=6000)
+    ap.add_argument("--batch", type=int, default=65536)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--summary", default=None, help="write a JSON summary here")
+    ap.add_argument("--compare-substeps", type=int, default=20)
+    args = ap.parse_args()
+
+    cap_particles = 125 ** 3
+    ob = workload_obstacle()
+    ctx = sa.Context(workload_params(), BOX, cap_particles)
+    ctx.set_obstacle(ob["sdf"], ob["dims"], ob["dx"], ob["wmin"], ob["wmax"])
+    xb = sa.DeviceBuffer(4 * sa.MLP_STRIDE * args.max_rows)
+    tb = sa.DeviceBuffer(4 * args.max_rows)
+    t0 = time.perf_counter()
+    rows, starts = generate(ctx, xb, tb, args.max_rows, args.scenes, args.seed, args.substeps,
+                            args.log_every, args.every)
+    t_gen = time.perf_counter() - t0
+    hold = starts[-args.holdout] if args.holdout and len(starts) > args.holdout else rows
+    print(f"[warm_train] {rows} rows ({hold} training) in {t_gen:.1f} s", flush=True)
+
+    net = sa.Mlp(seed=args.seed)
+    net.fit_normalization(xb, tb, hold)
+    t_train = train(net, xb, tb, hold, args.steps, args.batch, args.lr, args.seed)
+    train_loss, _ = net.gradient(xb, tb, hold)
+    val_loss = None
+    if hold < rows:
+        # validation rows: the held-out scenes, addressed from their first row
+        xv = sa.DeviceBuffer(4 * sa.MLP_STRIDE * (rows - hold))
+        sa.check(sa.lib().sph_device_copy(xv.ptr, _offset(xb, 4 * sa.MLP_STRIDE * hold),
+                                          4 * sa.MLP_STRIDE * (rows - hold), None))
+        tv = sa.DeviceBuffer.from_array(tb.to_array(np.float32, rows)[hold:])
+        val_loss, _ = net.gradient(xv, tv, rows - hold)
+    os.makedirs(os.path.dirname(args.out), exist_ok=True)
+    net.save(args.out)
+    summary = dict(rows=rows, training_rows=hold, scenes=len(starts), generate_s=t_gen,
+                   train_s=t_train, steps=args.steps, batch=args.batch,
+                   train_loss=train_loss, validation_loss=val_loss, out=args.out)
+    if args.compare_substeps:
+        # a held-out scene, mid-flight: plain vs warm at the workload's tolerance
+        rng = np.random.default_rng(args.seed * 1000 + 999)
+        pos = sa.scene_lattice([random_block(rng)], SPACING, jitter=0.01, seed=7)
+        ctx.upload(pos)
+        ctx.set_params(workload_params(10, 1e-3))
+        ctx.step(30)
+        p, v = ctx.download()
+        summary["compare"] = compare(ctx, net, p, v, workload_params(10, 1e-3),
+                                     args.compare_substeps)
+    print(json.dumps(summary), flush=True)
+    if args.summary:
+        with open(args.summary, "w") as f:
+            json.dump(summary, f)
+
+
+def _offset(buf, nbytes):
+    import ctypes
+    return ctypes.c_void_p(buf.ptr.value + int(nbytes))
+
+
+def compare(ctx, net, pos, vel, params, substeps, log=print):
+    """Plain vs warm-started tolerance solve from the same state: each mode
+    runs one untimed substep (which gives the warm start its previous
+    multiplier), then `substeps` substeps, each synchronised and timed."""
+    out = {}
+    for mode in ("plain", "warm"):
+        ctx.upload(pos, vel)
+        ctx.set_params(params)
+        ctx.set_warm_start(net if mode == "warm" else None)
+        ctx.step(1)
+        ctx.synchronize()
+        its, errs, conv = [], [], []
+        elapsed = 0.0
+        for _ in range(substeps):
+            t0 = time.perf_counter()
+            ctx.step(1)
+            ctx.synchronize()
+            elapsed += time.perf_counter() - t0
+            st = ctx.stats()
+            its.append(st.iterations)
+            errs.append(st.max_density_error)
+            conv.append(st.converged)
+        out[mode] = dict(iters_per_substep=float(np.mean(its)),
+                         max_density_error=float(np.max(errs)),
+                         converged_fraction=float(np.mean(conv)),
+                         ms_per_substep=1e3 * elapsed / substeps,
+                         particle_updates_per_s=ctx.count() * substeps / elapsed)
+        log(f"[warm_train] {mode}: {out[mode]}")
+    ctx.set_warm_start(None)
+    return out
+
+
+if __name__ == "__main__":
+    main()

@@ -68,3 +68,27 @@ def test_training_reduces_loss_on_a_smooth_target():
         l, g = M.loss_and_grad(theta, x, t)
         theta, m, v = M.adam(theta, m, v, g, step, lr=3e-3)
     assert l < 0.1 * l0
+
+
+def test_bf16_rounding_known_values():
+    v = np.array([1.0, 1 + 2 ** -8, 1 + 3 * 2 ** -9, 1 + 2 ** -7 + 2 ** -8, -2.5,
+                  3.0e-3, 65504.0])
+    r = M.bf16(v)
+    assert r[0] == 1.0
+    assert r[1] == 1.0                      # tie -> even mantissa
+    assert r[2] == 1 + 2 ** -7              # above the tie -> up
+    assert r[3] == 1 + 2 ** -6              # tie from an odd mantissa -> up to even
+    assert r[4] == -2.5
+    assert abs(r[5] - 3.0e-3) <= 3.0e-3 * 2 ** -8
+    assert r[6] == 65536.0                  # 65504 is closer to 2^16 at 8 bits
+    u = M.bf16(np.random.default_rng(0).normal(size=1000)).astype(np.float32).view(np.uint32)
+    assert np.all((u & 0xFFFF) == 0)
+
+
+def test_bf16_forward_close_to_fp64_forward():
+    theta = M.init(3)
+    xn = np.random.default_rng(5).normal(size=(256, M.N_IN))
+    y64 = M.forward(theta, xn)[0]
+    yb = M.forward_bf16(theta, xn)
+    assert np.max(np.abs(yb - y64)) < 0.05 * np.max(np.abs(y64))
+    assert np.max(np.abs(yb - y64)) > 0  # the rounding is really applied
