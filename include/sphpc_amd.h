@@ -142,6 +142,11 @@ int sph_update_positions(sph_ctx *ctx);            /* particles.c:566 */
 /* Whole substeps with fused kernels: the loop body of simulation.c:54-116
  * (output excluded), `substeps` times. */
 int sph_step(sph_ctx *ctx, int substeps);
+/* Replay whole substeps of sph_step from a HIP graph (captured after one eager
+ * substep; rebuilt after any change of parameters, obstacle, particle count,
+ * warm start, logging or profiling).  Results are bit-identical to the eager
+ * launches.  Off by default; timing with sph_profile_enable uses eager runs. */
+int sph_set_graph(sph_ctx *ctx, int enable);
 int sph_synchronize(sph_ctx *ctx);
 int sph_get_stats(sph_ctx *ctx, sph_stats *stats);
 /* FP64 reductions: kinetic energy sum(m/2 |v|^2) and max density error. */
@@ -254,6 +259,7 @@ typedef struct sph_run_config {
   double jitter;            /* lattice jitter in units of spacing (0) */
   unsigned long long seed;
   int output_every;         /* 0: int(1/(30 dt)) (simulation.c:113); < 0: never */
+  char warm_start_net[256]; /* trained warm-start network file ("" = none) */
 } sph_run_config;
 int sph_config_load(const char *path, sph_run_config *cfg);
 const char *sph_config_error(void);

@@ -138,6 +138,7 @@ def lib():
     L.sph_mlp_save.argtypes = [vp, C.c_char_p]
     L.sph_mlp_load.argtypes = [vp, C.c_char_p]
     L.sph_set_warm_start.argtypes = [vp, vp]
+    L.sph_set_graph.argtypes = [vp, ip]
     L.sph_set_sample_logging.argtypes = [vp, ip]
     L.sph_warm_samples.argtypes = [vp, vp, vp, sz, sz, ip, C.POINTER(sz)]
     _lib = L
@@ -278,6 +279,10 @@ class Context:
         e = C.c_double()
         check(lib().sph_compute_energy(self._h, C.byref(e)))
         return e.value
+
+    def set_graph(self, enable):
+        """Replay substeps from a captured HIP graph."""
+        check(lib().sph_set_graph(self._h, int(bool(enable))))
 
     def set_warm_start(self, net):
         """Warm-start following fused substeps with `net` (an Mlp), None = off."""

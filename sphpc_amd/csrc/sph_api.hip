@@ -152,7 +152,16 @@ static int check_params(const sph_params *p) {
   return SPH_OK;
 }
 
+static void graph_reset(sph_ctx *c) {
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->graph_exec = nullptr;
+  c->graph = nullptr;
+  c->graph_valid = false;
+}
+
 static void free_all(sph_ctx *c) {
+  graph_reset(c);
   void *bufs[] = {c->x_can, c->v_can, c->xs_can, c->keys, c->keys_alt, c->ids, c->ids_alt,
                   c->hist, c->cell_count, c->cell_start, c->pos_a, c->pos_b, c->xold,
                   c->vel_a, c->vel_b, c->aux, c->dpbuf, c->rho, c->list, c->list_cnt,
@@ -329,6 +338,7 @@ extern "C" int sph_set_params(sph_ctx *c, const sph_params *p) {
   if (rc) return rc;
   if (p->smoothing_radius != c->P.smoothing_radius)
     return arg_error("smoothing_radius fixes the hash grid; create a new context to change it");
+  graph_reset(c);
   c->P = *p;
   derive(c);
   return SPH_OK;
@@ -376,6 +386,7 @@ static int make_sdf(const sph_obstacle_desc *o, SdfDev *d, float **buf, hipStrea
 
 extern "C" int sph_set_obstacle(sph_ctx *c, const sph_obstacle_desc *o) {
   if (!c) return arg_error("context is NULL");
+  graph_reset(c);
   if (c->sdf_buf) {
     SPH_HIP(hipStreamSynchronize(c->stream));
     SPH_HIP(hipFree(c->sdf_buf));
@@ -402,6 +413,7 @@ extern "C" int sph_upload_state(sph_ctx *c, int n, const float *x, const float *
     return SPH_ERR_CAPACITY;
   }
   if (n > 0 && (!x || !y || !z || !vx || !vy || !vz)) return arg_error("NULL state array");
+  if (n != c->n) graph_reset(c);
   std::vector<float4> hx((size_t)n), hv((size_t)n);
   for (int i = 0; i < n; ++i) {
     hx[i] = make_float4(x[i], y[i], z[i], 0.0f);
@@ -542,7 +554,8 @@ extern "C" int sph_update_positions(sph_ctx *c) {
 // --------------------------------------------------------------------------
 // Fused substep
 // --------------------------------------------------------------------------
-static int substep_fused(sph_ctx *c) {
+// Device work of one fused substep, all on the context stream (capturable).
+static int substep_launch(sph_ctx *c, bool *warm_out) {
   int rc = launch_substep_init(c);
   if (rc) return rc;
   rc = launch_neighbor_search(c, true);
@@ -552,6 +565,7 @@ static int substep_fused(sph_ctx *c) {
   // computes the network features (and, warm, replaces lambda by the
   // network's accumulated multiplier); later iterations are the plain ones.
   const bool warm = iters > 0 && (c->log_on || c->warm_on);
+  *warm_out = warm;
   int it0 = 0;
   if (warm) {
     if ((rc = launch_warm_lambda(c, c->log_on))) return rc;
@@ -582,8 +596,61 @@ static int substep_fused(sph_ctx *c) {
     if ((rc = launch_commit(c))) return rc;
   }
   if (warm && (rc = launch_scatter_lambda(c))) return rc;
+  return SPH_OK;
+}
+
+static void substep_finish(sph_ctx *c, bool warm) {
   c->substeps++;
   if (warm && c->log_on) c->feat_substep = c->substeps;
+}
+
+static int substep_fused(sph_ctx *c) {
+  bool warm = false;
+  int rc = substep_launch(c, &warm);
+  if (rc) return rc;
+  substep_finish(c, warm);
+  return SPH_OK;
+}
+
+// Records one substep into a graph without running it.  Every lazily sized
+// buffer is allocated by then (an eager substep of the same size ran first),
+// so the capture contains kernel launches and no allocation or memset.
+static int capture_substep(sph_ctx *c) {
+  graph_reset(c);
+  SPH_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  bool warm = false;
+  const int rc = substep_launch(c, &warm);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (rc || e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc ? rc : hip_fail(e, "hipStreamEndCapture");
+  }
+  const hipError_t ei = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+  if (ei != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    c->graph_exec = nullptr;
+    return hip_fail(ei, "hipGraphInstantiate");
+  }
+  c->graph = g;
+  c->graph_after = {c->pos, c->vel, c->key_s, c->id_s, c->ntiles, c->pair_blocks,
+                    c->last_err_slot, warm};
+  c->graph_valid = true;
+  return SPH_OK;
+}
+
+static int substep_graph(sph_ctx *c) {
+  SPH_HIP(hipGraphLaunch(c->graph_exec, c->stream));
+  const auto &a = c->graph_after;
+  c->pos = a.pos;
+  c->vel = a.vel;
+  c->key_s = a.key_s;
+  c->id_s = a.id_s;
+  c->ntiles = a.ntiles;
+  c->pair_blocks = a.pair_blocks;
+  c->last_err_slot = a.last_err_slot;
+  c->sorted_valid = c->lists_valid = true;
+  substep_finish(c, a.warm);
   return SPH_OK;
 }
 
@@ -603,6 +670,7 @@ static int ensure_warm_buffers(sph_ctx *c, bool features) {
 
 extern "C" int sph_set_warm_start(sph_ctx *c, sph_mlp *net) {
   if (!c) return arg_error("context is NULL");
+  graph_reset(c);
   if (!net) {
     c->warm_on = false;
     return SPH_OK;
@@ -619,6 +687,7 @@ extern "C" int sph_set_warm_start(sph_ctx *c, sph_mlp *net) {
 
 extern "C" int sph_set_sample_logging(sph_ctx *c, int enable) {
   if (!c) return arg_error("context is NULL");
+  graph_reset(c);
   if (enable) {
     int rc = ensure_warm_buffers(c, true);
     if (rc) return rc;
@@ -650,9 +719,21 @@ extern "C" int sph_step(sph_ctx *c, int substeps) {
   if (rc) return rc;
   if (substeps < 0) return arg_error("substeps must be >= 0");
   for (int s = 0; s < substeps; ++s) {
-    rc = substep_fused(c);
+    if (c->graph_on && !c->prof.on && c->graph_valid) {
+      rc = substep_graph(c);
+    } else {
+      rc = substep_fused(c);
+      if (!rc && c->graph_on && !c->prof.on && c->n > 0) rc = capture_substep(c);
+    }
     if (rc) return rc;
   }
+  return SPH_OK;
+}
+
+extern "C" int sph_set_graph(sph_ctx *c, int enable) {
+  if (!c) return arg_error("context is NULL");
+  graph_reset(c);
+  c->graph_on = enable != 0;
   return SPH_OK;
 }
 
@@ -886,6 +967,7 @@ extern "C" int sph_profile_enable(sph_ctx *c, int enable) {
   if (!c) return arg_error("context is NULL");
   if (!enable) prof_collect(c);
   c->prof.on = enable != 0;
+  graph_reset(c);
   return SPH_OK;
 }
 

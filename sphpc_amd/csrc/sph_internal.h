@@ -166,6 +166,18 @@ struct sph_ctx {
   bool warm_on = false;
   bool log_on = false;
   long long feat_substep = -1;  // substep count after the last logged substep
+  // whole-substep HIP graph (sph_set_graph)
+  bool graph_on = false;
+  bool graph_valid = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  struct {
+    float4 *pos, *vel;
+    const uint32_t *key_s, *id_s;
+    uint32_t ntiles, pair_blocks;
+    int last_err_slot;
+    bool warm;
+  } graph_after{};
   int last_err_slot = -1;
   int staged_slot = 0;
   bool state_valid = false;

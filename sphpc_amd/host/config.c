@@ -173,6 +173,9 @@ int sph_config_load(const char *path, sph_run_config *cfg) {
   rc |= get_double(E, n, "Solver.jitter", &cfg->jitter, 0);
   rc |= get_double(E, n, "Solver.seed", &seed, 0);
   rc |= get_int(E, n, "Solver.output_every", &cfg->output_every, 0);
+  cfg->warm_start_net[0] = '\0';
+  const char *net = find(E, n, "Solver.warm_start_net");
+  if (net) snprintf(cfg->warm_start_net, sizeof(cfg->warm_start_net), "%s", net);
   cfg->seed = (unsigned long long)seed;
   return rc ? SPH_ERR_ARG : SPH_OK;
 }

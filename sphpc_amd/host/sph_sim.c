@@ -151,6 +151,15 @@ int main(int argc, char **argv) {
   int frame = 0;
   if (output && cfg.output_every >= 0) write_frame(frame++, n, x, y, z);
 
+  /* Learned warm start of the fused solve (no counterpart in the reference). */
+  sph_mlp *net = NULL;
+  if (fused && cfg.warm_start_net[0]) {
+    CHECK(sph_mlp_create(&net));
+    CHECK(sph_mlp_load(net, cfg.warm_start_net));
+    CHECK(sph_set_warm_start(ctx, net));
+    printf("Warm start: %s\n", cfg.warm_start_net);
+  }
+
   const double start = now();
   for (int step = 0; step < cfg.number_steps; ++step) {
     if (fused) {
@@ -186,6 +195,7 @@ int main(int argc, char **argv) {
   printf("particles %d  substeps %lld  last max density error %.6f  neighbour overflow %llu\n",
          st.particles, st.substeps, st.max_density_error, st.neighbor_overflow);
   CHECK(sph_destroy(ctx));
+  if (net) CHECK(sph_mlp_destroy(net));
   free(buf);
   return 0;
 }

@@ -1,0 +1,84 @@
+"""Whole-substep HIP graph replay against eager launches.
+
+Every kernel of the substep is deterministic (integer atomics and max
+reductions only), so a replayed graph must give bit-identical states and
+statistics.  The graph is replayed several times with eager work and host
+synchronisations in between, and every replay is compared."""
+import numpy as np
+import pytest
+
+from gpu_common import Scene, block_scene
+
+pytestmark = pytest.mark.gpu
+
+sa = pytest.importorskip("sphpc_amd")
+
+
+def _pair(tol, iters=4):
+    s = 0.02
+    sc = Scene(s, (2.0, 1.0, 1.0), iters=iters, tol=tol, k=0.5, c=0.01, vort_eps=0.01,
+               vort_model=1, lambda_eps=600.0)
+    pos, vel = block_scene((24, 20, 22), s, (0.1, 0.05, 0.1), jitter=0.01, seed=3,
+                           vel_sigma=0.1, swirl=0.3)
+    ctxs = []
+    for graph in (False, True):
+        c = sc.context(len(pos))
+        dims, wmin, wmax = (32, 32, 32), (0.5, 0.0, 0.0), (1.3, 0.8, 0.8)
+        ax = [wmin[i] + (np.arange(32) + 0.5) * 0.025 for i in range(3)]
+        Z, Y, X = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+        sdf = (np.sqrt((X - 0.9) ** 2 + (Y - 0.3) ** 2 + (Z - 0.2) ** 2) - 0.15).astype(np.float32)
+        c.set_obstacle(sdf.ravel(), dims, 0.025, wmin, wmax)
+        c.upload(pos, vel)
+        c.set_graph(graph)
+        ctxs.append(c)
+    return sc, ctxs
+
+
+def _same(a, b, step):
+    pa, va = a.download()
+    pb, vb = b.download()
+    assert np.array_equal(pa, pb), step
+    assert np.array_equal(va, vb), step
+    sa_, sb = a.stats(), b.stats()
+    assert sa_.iterations == sb.iterations, step
+    assert sa_.converged == sb.converged, step
+    assert sa_.max_density_error == sb.max_density_error, step
+    assert sa_.neighbor_pairs == sb.neighbor_pairs, step
+    assert sa_.substeps == sb.substeps, step
+
+
+@pytest.mark.parametrize("tol", [0.0, 1e-3])
+def test_graph_replay_bit_identical(tol):
+    sc, (eager, graph) = _pair(tol)
+    for step in range(6):
+        eager.step(1)
+        graph.step(1)
+        graph.synchronize()
+        _same(eager, graph, step)
+        # eager work on the graph context between replays (FP64 block sums
+        # are added atomically in any order: equal to rounding)
+        ke_g, ke_e = graph.kinetic_energy(), eager.kinetic_energy()
+        assert abs(ke_g - ke_e) <= 1e-12 * max(abs(ke_e), 1e-300)
+    # a parameter change rebuilds the graph
+    p = sc.gpu_params()
+    p.solve_iterations = 6
+    for c in (eager, graph):
+        c.set_params(p)
+    eager.step(3)
+    graph.step(3)
+    _same(eager, graph, "after set_params")
+
+
+def test_graph_replay_with_warm_start():
+    sc, (eager, graph) = _pair(1e-3, iters=10)
+    net = sa.Mlp(seed=4)
+    mean = np.array([1.0, 0.0, 0.01, 0.9, 0.0, 4.0, 0, 0, 0, 0, 0, 0], np.float32)
+    std = np.array([0.3, 0.01, 0.01, 0.05, 0.01, 1.0, 1, 1, 1, 0.1, 0.1, 0.1], np.float32)
+    net.set_normalization(mean, std, 0.005, 0.01)
+    for c in (eager, graph):
+        c.set_warm_start(net)
+    for step in range(5):
+        eager.step(1)
+        graph.step(1)
+        _same(eager, graph, step)
+        assert np.array_equal(eager.field(sa.FIELD_LAMBDA_SUM), graph.field(sa.FIELD_LAMBDA_SUM))

@@ -114,7 +114,8 @@ class RunConfig(C.Structure):
                 ("vorticity_epsilon", C.c_double), ("mass_factor", C.c_double),
                 ("density_tolerance", C.c_double), ("vorticity_model", C.c_int),
                 ("surface_tension", C.c_int), ("viscosity", C.c_int), ("vorticity", C.c_int),
-                ("jitter", C.c_double), ("seed", C.c_ulonglong), ("output_every", C.c_int)]
+                ("jitter", C.c_double), ("seed", C.c_ulonglong), ("output_every", C.c_int),
+                ("warm_start_net", C.c_char * 256)]
 
 
 REFERENCE_INI = """[SimParameters]
@@ -194,6 +195,14 @@ def test_config_optional_obstacle_and_solver_keys(tmp_path):
     assert cfg.solve_iterations == 4 and cfg.lambda_epsilon == 600.0
     assert cfg.mass_factor == 1.0 and cfg.density_tolerance == 0.001
     assert cfg.vorticity_model == 1 and cfg.jitter == 0.01 and cfg.seed == 42
+    assert cfg.warm_start_net == b""
+
+
+def test_config_warm_start_net_key(tmp_path):
+    p = tmp_path / "sim-config.ini"
+    p.write_text(REFERENCE_INI + "\n[Solver]\nwarm_start_net = nets/warm.bin\n")
+    rc, cfg, _ = _load(p)
+    assert rc == 0 and cfg.warm_start_net == b"nets/warm.bin"
 
 
 def test_reference_runtime_ini_particle_count():
