@@ -182,8 +182,7 @@ def main():
                                "surface tension + XSPH + vorticity",
                    "particles": n, "scale": args.scale, "mean_neighbors": nbar,
                    "max_density_error_last": st.max_density_error,
-                   "neighbor_overflow": st.neighbor_overflow,
-                   "tiles": st.tiles, "tiles_global": st.tiles_global},
+                   "neighbor_overflow": st.neighbor_overflow},
     }
     gathers = {k: v for k, v in times.items() if k in KERNEL_BYTES}
     dom = max(gathers, key=lambda k: gathers[k][0])
@@ -236,9 +235,15 @@ def main():
         out["jacobi"] = {"tolerance": 1e-3, "cap": 10, "substeps": args.tol_steps,
                          "plain_iters_per_substep": pl["iters_per_substep"],
                          "plain_max_density_error": pl["max_density_error"],
+                         "plain_mean_final_error": pl.get("mean_final_error"),
+                         "plain_error_after_iterations": pl.get("error_after_iterations"),
                          "plain_particle_updates_per_s": pl["particle_updates_per_s"],
                          "warm_iters_per_substep": wm["iters_per_substep"] if wm else None,
                          "warm_max_density_error": wm["max_density_error"] if wm else None,
+                         "warm_mean_final_error": wm["mean_final_error"] if wm else None,
+                         "warm_error_after_iterations":
+                             wm["error_after_iterations"] if wm else None,
+                         "warm_fallbacks": wm["fallbacks"] if wm else None,
                          "warm_particle_updates_per_s":
                              wm["particle_updates_per_s"] if wm else None,
                          "warm_speedup": (wm["particle_updates_per_s"] /

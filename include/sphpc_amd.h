@@ -93,8 +93,7 @@ typedef struct sph_stats {
   unsigned long long cell_clamps;       /* cell coordinates clamped into the grid */
   long long substeps;        /* substeps run on this context */
   unsigned long long neighbor_pairs;    /* sum of list lengths of the last search */
-  int tiles;                 /* neighbour tiles of the last search */
-  int tiles_global;          /* tiles whose neighbourhood did not fit in LDS */
+  int warm_fallback;         /* warm start reverted to the plain solve (last substep) */
 } sph_stats;
 
 typedef struct sph_ctx sph_ctx;
@@ -149,6 +148,10 @@ int sph_step(sph_ctx *ctx, int substeps);
 int sph_set_graph(sph_ctx *ctx, int enable);
 int sph_synchronize(sph_ctx *ctx);
 int sph_get_stats(sph_ctx *ctx, sph_stats *stats);
+/* Max density error of every density pass of the last fused substep:
+ * err[k] = max_i max(rho_i - rho0, 0) / rho0 after k position updates
+ * (*count passes, at most cap). */
+int sph_get_residuals(sph_ctx *ctx, float *err, int cap, int *count);
 /* FP64 reductions: kinetic energy sum(m/2 |v|^2) and max density error. */
 int sph_compute_energy(sph_ctx *ctx, double *kinetic_energy);
 
@@ -313,6 +316,10 @@ int sph_mlp_get_normalization(sph_mlp *net, float *mean, float *stddev, float *t
                               float *target_std);
 int sph_mlp_fit_normalization(sph_mlp *net, const float *x_dev, const float *t_dev,
                               size_t count);
+/* Caps the output used by the solver at max_target (0: no cap); saved with
+ * the network.  Training clips its targets at the same bound. */
+int sph_mlp_set_output_clip(sph_mlp *net, float max_target);
+float sph_mlp_get_output_clip(sph_mlp *net);
 /* Full-batch gradient of 0.5/B sum (y_n - t_n)^2 in normalised target units
  * (host outputs; either may be NULL). */
 int sph_mlp_gradient(sph_mlp *net, const float *x_dev, const float *t_dev, size_t count,
@@ -331,7 +338,9 @@ int sph_mlp_save(sph_mlp *net, const char *path);
 int sph_mlp_load(sph_mlp *net, const char *path);
 
 /* Warm-start the solve of every following fused substep with `net` (its
- * current weights are copied; NULL turns the warm start off). */
+ * current weights are copied; NULL turns the warm start off).  If the
+ * predicted step raises the maximum density error, the substep falls back to
+ * the plain first iteration (sph_stats.warm_fallback). */
 int sph_set_warm_start(sph_ctx *ctx, sph_mlp *net);
 /* Record features and accumulated multipliers of every following fused
  * substep (plain solve; takes precedence over the warm start). */

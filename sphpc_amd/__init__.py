@@ -57,8 +57,7 @@ class Stats(C.Structure):
     _fields_ = [("particles", C.c_int), ("iterations", C.c_int), ("converged", C.c_int),
                 ("max_density_error", C.c_float), ("neighbor_overflow", C.c_ulonglong),
                 ("cell_clamps", C.c_ulonglong), ("substeps", C.c_longlong),
-                ("neighbor_pairs", C.c_ulonglong), ("tiles", C.c_int),
-                ("tiles_global", C.c_int)]
+                ("neighbor_pairs", C.c_ulonglong), ("warm_fallback", C.c_int)]
 
 
 _lib = None
@@ -139,6 +138,10 @@ def lib():
     L.sph_mlp_load.argtypes = [vp, C.c_char_p]
     L.sph_set_warm_start.argtypes = [vp, vp]
     L.sph_set_graph.argtypes = [vp, ip]
+    L.sph_get_residuals.argtypes = [vp, fp, ip, C.POINTER(ip)]
+    L.sph_mlp_set_output_clip.argtypes = [vp, C.c_float]
+    L.sph_mlp_get_output_clip.argtypes = [vp]
+    L.sph_mlp_get_output_clip.restype = C.c_float
     L.sph_set_sample_logging.argtypes = [vp, ip]
     L.sph_warm_samples.argtypes = [vp, vp, vp, sz, sz, ip, C.POINTER(sz)]
     _lib = L
@@ -280,6 +283,13 @@ class Context:
         check(lib().sph_compute_energy(self._h, C.byref(e)))
         return e.value
 
+    def residuals(self):
+        """Max density error of each density pass of the last fused substep."""
+        buf = np.zeros(160, np.float32)
+        k = C.c_int()
+        check(lib().sph_get_residuals(self._h, _fptr(buf), len(buf), C.byref(k)))
+        return buf[:k.value].copy()
+
     def set_graph(self, enable):
         """Replay substeps from a captured HIP graph."""
         check(lib().sph_set_graph(self._h, int(bool(enable))))
@@ -393,6 +403,12 @@ class Mlp:
         check(lib().sph_mlp_get_normalization(self._h, _fptr(m), _fptr(s), C.byref(tm),
                                               C.byref(ts)))
         return m, s, tm.value, ts.value
+
+    def set_output_clip(self, max_target):
+        check(lib().sph_mlp_set_output_clip(self._h, float(max_target)))
+
+    def output_clip(self):
+        return lib().sph_mlp_get_output_clip(self._h)
 
     def fit_normalization(self, x_buf, t_buf, count):
         check(lib().sph_mlp_fit_normalization(self._h, x_buf.ptr, t_buf.ptr, int(count)))

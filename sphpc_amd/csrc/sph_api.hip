@@ -166,7 +166,7 @@ static void free_all(sph_ctx *c) {
                   c->hist, c->cell_count, c->cell_start, c->pos_a, c->pos_b, c->xold,
                   c->vel_a, c->vel_b, c->aux, c->dpbuf, c->rho, c->list, c->list_cnt,
                   c->stats, c->pair_partial, c->red, c->sdf_buf, c->scan.tile_sums,
-                  c->list16, c->tile_meta, c->tile_int, c->lam_acc, c->lam_prev, c->feat,
+                  c->pw, c->lam_acc, c->lam_prev, c->feat,
                   c->warm_net};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
@@ -285,8 +285,7 @@ extern "C" int sph_create(const sph_params *params, const sph_aabb *boundary, in
 
   const size_t cap = (size_t)std::max(max_particles, 1);
   const size_t f4 = sizeof(float4) * cap, u4 = sizeof(uint32_t) * cap;
-  const size_t tiles = (cap + kTile - 1) / kTile;
-  const size_t groups = tiles * (kTile / kWave);  // list groups cover whole tiles
+  const size_t groups = (cap + kWave - 1) / kWave;
   struct {
     void **p;
     size_t bytes;
@@ -301,12 +300,10 @@ extern "C" int sph_create(const sph_params *params, const sph_aabb *boundary, in
       {(void **)&c->vel_b, f4},       {(void **)&c->aux, f4},
       {(void **)&c->dpbuf, f4},       {(void **)&c->rho, sizeof(float) * cap},
       {(void **)&c->list, sizeof(uint32_t) * groups * kListStride},
-      {(void **)&c->list16, sizeof(uint16_t) * groups * kListStride},
       {(void **)&c->list_cnt, sizeof(int) * cap},
-      {(void **)&c->tile_meta, sizeof(int2) * tiles},
-      {(void **)&c->tile_int, sizeof(uint2) * tiles * kMaxInt},
+      {(void **)&c->pw, f4},
       {(void **)&c->stats, sizeof(uint32_t) * STAT_WORDS},
-      {(void **)&c->pair_partial, sizeof(uint32_t) * tiles},
+      {(void **)&c->pair_partial, sizeof(uint32_t) * blocks_for((uint32_t)cap)},
       {(void **)&c->red, sizeof(double) * 4},
   };
   for (auto &a : allocs) {
@@ -567,13 +564,16 @@ static int substep_launch(sph_ctx *c, bool *warm_out) {
   const bool warm = iters > 0 && (c->log_on || c->warm_on);
   *warm_out = warm;
   int it0 = 0;
+  const float4 *pre_warm = nullptr;
   if (warm) {
     if ((rc = launch_warm_lambda(c, c->log_on))) return rc;
+    pre_warm = c->pos;
     if ((rc = launch_dp_update(c, 0, 0, true))) return rc;
     it0 = 1;
   }
   for (int it = it0; it < iters; ++it) {
     if ((rc = launch_density(c, 0, it))) return rc;
+    if (it == 1 && warm && !c->log_on && (rc = launch_warm_fallback(c, pre_warm))) return rc;
     if ((rc = launch_dp_update(c, it, it, warm))) return rc;
   }
   c->last_err_slot = iters > 0 ? iters - 1 : -1;
@@ -633,7 +633,7 @@ static int capture_substep(sph_ctx *c) {
     return hip_fail(ei, "hipGraphInstantiate");
   }
   c->graph = g;
-  c->graph_after = {c->pos, c->vel, c->key_s, c->id_s, c->ntiles, c->pair_blocks,
+  c->graph_after = {c->pos, c->vel, c->key_s, c->id_s, c->pair_blocks,
                     c->last_err_slot, warm};
   c->graph_valid = true;
   return SPH_OK;
@@ -646,7 +646,6 @@ static int substep_graph(sph_ctx *c) {
   c->vel = a.vel;
   c->key_s = a.key_s;
   c->id_s = a.id_s;
-  c->ntiles = a.ntiles;
   c->pair_blocks = a.pair_blocks;
   c->last_err_slot = a.last_err_slot;
   c->sorted_valid = c->lists_valid = true;
@@ -752,6 +751,7 @@ extern "C" int sph_get_stats(sph_ctx *c, sph_stats *st) {
   st->particles = c->n;
   st->iterations = (int)w[STAT_ITERS];
   st->converged = (int)w[STAT_CONVERGED];
+  st->warm_fallback = (int)w[STAT_FALLBACK];
   int slot = c->last_err_slot;
   if (c->P.density_tolerance > 0.0 && st->converged) slot = st->iterations;
   if (slot >= 0 && slot <= kMaxIters) {
@@ -764,8 +764,6 @@ extern "C" int sph_get_stats(sph_ctx *c, sph_stats *st) {
   st->cell_clamps =
       (unsigned long long)w[STAT_CLAMPS] | ((unsigned long long)w[STAT_CLAMPS + 1] << 32);
   st->substeps = c->substeps;
-  st->tiles = (int)c->ntiles;
-  st->tiles_global = (int)w[STAT_GLOBAL_TILES];
   if (c->pair_blocks) {
     std::vector<uint32_t> part;
     int rc = fetch(c, (const uint32_t *)c->pair_partial, c->pair_blocks, &part);
@@ -774,6 +772,21 @@ extern "C" int sph_get_stats(sph_ctx *c, sph_stats *st) {
     for (uint32_t v : part) sum += v;
     st->neighbor_pairs = sum;
   }
+  return SPH_OK;
+}
+
+extern "C" int sph_get_residuals(sph_ctx *c, float *err, int cap, int *count) {
+  if (!c || !err || !count) return arg_error("NULL argument");
+  uint32_t w[STAT_WORDS];
+  SPH_HIP(hipMemcpyAsync(w, c->stats, sizeof(w), hipMemcpyDeviceToHost, c->stream));
+  SPH_HIP(hipStreamSynchronize(c->stream));
+  // one density pass before each applied update; with a tolerance, one more
+  // (the pass that met it, or the final pass after the capped loop)
+  const int applied = (int)w[STAT_ITERS];
+  int passes = c->P.density_tolerance > 0.0 ? applied + 1 : applied;
+  passes = std::max(0, std::min(passes, std::min(cap, kMaxIters + 1)));
+  for (int k = 0; k < passes; ++k) memcpy(&err[k], &w[STAT_ERR0 + k], sizeof(float));
+  *count = passes;
   return SPH_OK;
 }
 
@@ -843,37 +856,18 @@ extern "C" int sph_download_neighbors(sph_ctx *c, uint32_t *out_ids, int *counts
   if (rc) return rc;
   if (!out_ids || !counts) return arg_error("NULL argument");
   const int n = c->n;
-  const size_t tiles = c->ntiles;
-  const size_t groups = tiles * (kTile / kWave);
+  const size_t groups = ((size_t)n + kWave - 1) / kWave;
   std::vector<uint32_t> ids, list;
-  std::vector<uint16_t> list16;
   std::vector<int> cnt;
-  std::vector<int2> meta;
-  std::vector<uint2> ints;
   if ((rc = fetch(c, c->id_s, (size_t)n, &ids))) return rc;
   if ((rc = fetch(c, (const int *)c->list_cnt, (size_t)n, &cnt))) return rc;
   if ((rc = fetch(c, (const uint32_t *)c->list, groups * kListStride, &list))) return rc;
-  if ((rc = fetch(c, (const uint16_t *)c->list16, groups * kListStride, &list16))) return rc;
-  if ((rc = fetch(c, (const int2 *)c->tile_meta, tiles, &meta))) return rc;
-  if ((rc = fetch(c, (const uint2 *)c->tile_int, tiles * kMaxInt, &ints))) return rc;
   for (int s = 0; s < n; ++s) {
     const uint32_t id = ids[s];
-    const size_t t = (size_t)s / kTile;
-    const int2 m = meta[t];
     counts[id] = cnt[s];
     const size_t base = (size_t)(s >> 6) * kListStride + (s & 63);
-    for (int k = 0; k < cnt[s]; ++k) {
-      uint32_t j;
-      if (m.x >= 0) {  // staged tile: LDS slot -> sorted index through the staging map
-        const uint32_t slot = list16[base + (size_t)k * kWave];
-        int i = m.x - 1;
-        while (i > 0 && ints[t * kMaxInt + i].y > slot) --i;
-        j = ints[t * kMaxInt + i].x + (slot - ints[t * kMaxInt + i].y);
-      } else {
-        j = list[base + (size_t)k * kWave];
-      }
-      out_ids[(size_t)id * kMaxNb + k] = ids[j];
-    }
+    for (int k = 0; k < cnt[s]; ++k)
+      out_ids[(size_t)id * kMaxNb + k] = ids[list[base + (size_t)k * kWave]];
   }
   return SPH_OK;
 }

@@ -16,15 +16,6 @@ constexpr int kBlock = 256;          // threads per block for elementwise kernel
 constexpr int kWave = 64;            // CDNA wavefront
 constexpr int kListStride = kWave * kMaxNb;  // list entries per 64-particle group
 
-// Neighbour tiles: kTile consecutive sorted particles form one workgroup of the
-// list build and of every gather pass.  The union of the tile's neighbour row
-// ranges (at most kMaxInt index intervals, at most kCap particles) is staged in
-// LDS and the tile's lists hold 16-bit LDS slots.  A tile that does not fit
-// keeps 32-bit global indices and gathers from global memory.
-constexpr int kTile = 128;
-constexpr int kCap = 2048;
-constexpr int kMaxInt = 32;
-constexpr int kMaxSeg = 16;          // merged intervals per (dz, dy) row in the build
 
 // Device-side constants of one run, all FP32 except the exact-predicate h^2.
 struct DevParams {
@@ -68,7 +59,7 @@ enum StatSlot {
   STAT_CLAMPS = 2,       // 64-bit counter (two words)
   STAT_ITERS = 4,
   STAT_CONVERGED = 5,
-  STAT_GLOBAL_TILES = 6, // tiles whose neighbourhood did not fit in LDS
+  STAT_FALLBACK = 6,     // warm start reverted to the plain first iteration
   STAT_ERR0 = 8,         // per-iteration max error bits (kMaxIters + 1 slots)
   STAT_WORDS = 256
 };
@@ -82,7 +73,9 @@ constexpr int kMlpW1 = 0, kMlpB1 = kMlpW1 + kMlpPad * kMlpH, kMlpW2 = kMlpB1 + k
               kMlpB2 = kMlpW2 + kMlpH * kMlpH, kMlpW3 = kMlpB2 + kMlpH,
               kMlpB3 = kMlpW3 + kMlpH, kMlpParams = kMlpB3 + 1;  // 5313
 // Normalisation words: feature means, reciprocal deviations, target shift/scale.
-constexpr int kNormMu = 0, kNormIsd = 16, kNormYShift = 32, kNormYScale = 33, kNormWords = 36;
+// kNormYMax > 0 caps the network output (the clipping bound of its training targets).
+constexpr int kNormMu = 0, kNormIsd = 16, kNormYShift = 32, kNormYScale = 33, kNormYMax = 34,
+              kNormWords = 36;
 
 // BF16 inference weights packed in MFMA operand order (layout in sph_mlp_dev.h).
 struct MlpPacked {
@@ -146,14 +139,11 @@ struct sph_ctx {
   float4 *xold = nullptr;
   float4 *vel_a = nullptr, *vel_b = nullptr, *vel = nullptr;  // vel: current v
   float4 *aux = nullptr;  // colour normal, then vorticity (w = |omega|)
+  float4 *pw = nullptr;   // (x*, |omega|) for the confinement gather
   float4 *dpbuf = nullptr;  // staged delta p
   float *rho = nullptr;
-  uint32_t *list = nullptr;     // 32-bit sorted indices (tiles in global mode)
-  uint16_t *list16 = nullptr;   // 16-bit LDS slots (staged tiles)
+  uint32_t *list = nullptr;     // sorted neighbour indices, slot-major per 64 particles
   int *list_cnt = nullptr;
-  int2 *tile_meta = nullptr;    // (intervals, staged count); intervals < 0: global mode
-  uint2 *tile_int = nullptr;    // kMaxInt (start index, LDS offset) per tile
-  uint32_t ntiles = 0;
   uint32_t *stats = nullptr;  // sph::STAT_WORDS words
   uint32_t *pair_partial = nullptr;  // per-block list-length sums of the last search
   uint32_t pair_blocks = 0;
@@ -174,7 +164,7 @@ struct sph_ctx {
   struct {
     float4 *pos, *vel;
     const uint32_t *key_s, *id_s;
-    uint32_t ntiles, pair_blocks;
+    uint32_t pair_blocks;
     int last_err_slot;
     bool warm;
   } graph_after{};
@@ -236,6 +226,7 @@ int launch_kinetic_energy(sph_ctx *c, double *out_dev);
 int launch_warm_lambda(sph_ctx *c, bool log);
 int launch_scatter_lambda(sph_ctx *c);
 int launch_warm_samples(sph_ctx *c, float *x, float *t, uint32_t count, int every);
+int launch_warm_fallback(sph_ctx *c, const float4 *pre_warm);
 int mlp_pack(sph_mlp *m, MlpPacked *dst, hipStream_t s);  // sph_mlp.hip
 int configure_kernels();  // LDS limits of the tiled kernels, once per process
 

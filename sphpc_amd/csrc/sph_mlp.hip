@@ -512,6 +512,15 @@ extern "C" int sph_mlp_set_normalization(sph_mlp *m, const float *mean, const fl
   return upload_norm(m);
 }
 
+extern "C" int sph_mlp_set_output_clip(sph_mlp *m, float max_target) {
+  if (!m) return mlp_arg("NULL argument");
+  if (!(max_target >= 0.0f)) return mlp_arg("max_target must be >= 0 (0: no clip)");
+  m->host_norm[kNormYMax] = max_target;
+  return upload_norm(m);
+}
+
+extern "C" float sph_mlp_get_output_clip(sph_mlp *m) { return m ? m->host_norm[kNormYMax] : 0.0f; }
+
 extern "C" int sph_mlp_get_normalization(sph_mlp *m, float *mean, float *stddev,
                                          float *target_mean, float *target_std) {
   if (!m) return mlp_arg("NULL argument");

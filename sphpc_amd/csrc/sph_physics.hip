@@ -3,22 +3,16 @@
 // Particles live in two orders.  The canonical state (x_can, v_can) is in
 // particle-id order, like the reference's arrays.  After the neighbour search
 // the substep works on copies reordered into (cell key, id) order, so that a
-// particle's 27-cell neighbourhood is nine contiguous index ranges (the three
-// x-cells of a (dz, dy) row have consecutive keys).  The commit kernel scatters
-// x* and v back to id order at the end of the substep.
+// particle's 27-cell neighbourhood is nine contiguous index ranges and the
+// neighbour gathers of one wavefront hit the same cache lines.  The commit
+// kernel scatters x* and v back to id order at the end of the substep.
 //
 // Vector fields are float4.  pos.w carries lambda between the density/lambda
 // pass and the delta-p pass, and rho afterwards; aux.w carries |omega|.
 //
-// Neighbour tiles.  kTile consecutive sorted particles are one workgroup of
-// the list build and of every gather pass.  The list build merges the tile's
-// 9 x kTile neighbour row ranges into at most kMaxInt index intervals
-// (staging map); when they hold at most kCap particles the tile is "staged":
-// each gather pass copies those particles into LDS with coalesced loads and
-// the lists hold 16-bit LDS slots, so neighbour reads never leave the CU.
-// Other tiles keep 32-bit sorted indices and gather from global memory.
-// Lists are slot-major per 64-particle group: entry k of particle s is at
-// (s/64)*64*60 + k*64 + s%64, so a wave reading entry k reads contiguously.
+// Neighbour lists are stored per 64-particle group, slot-major
+// (list[(s/64)*64*60 + k*64 + s%64]), so lane s of a wave reading its k-th
+// neighbour index reads 64 consecutive words.
 #include "sph_internal.h"
 #include "sph_mlp_dev.h"
 
@@ -33,26 +27,28 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
   return x * q + min(x, r) + idx;
 }
 
-template <typename Idx>
-__device__ __forceinline__ const Idx *list_ptr(const Idx *list, uint32_t s) {
+__device__ __forceinline__ uint32_t particle_index(uint32_t n_blocks) {
+  return xcd_block(blockIdx.x, n_blocks) * kBlock + threadIdx.x;
+}
+
+__device__ __forceinline__ const uint32_t *list_ptr(const uint32_t *list, uint32_t s) {
   return list + (size_t)(s >> 6) * kListStride + (s & 63u);
 }
 
 // Visits the c neighbours of a particle in list order.  Each round first
-// issues kBatch list loads, then the kBatch dependent loads, then the
+// issues kBatch list loads, then the kBatch dependent gathers, then the
 // arithmetic, so every wave keeps kBatch loads in flight instead of one
-// dependent chain per neighbour.  Padding slots load index `pad` (always
-// valid) and are skipped by the body.
+// dependent chain per neighbour.  Padding slots load the particle itself
+// (`self`, always a valid index) and are skipped by the body.
 constexpr int kBatch = 8;
 
-template <typename T, typename Idx, typename Load, typename Body>
-__device__ __forceinline__ void for_neighbors(const Idx *L, int c, uint32_t pad, Load load,
+template <typename T, typename Load, typename Body>
+__device__ __forceinline__ void for_neighbors(const uint32_t *L, int c, uint32_t self, Load load,
                                               Body body) {
   for (int k0 = 0; k0 < c; k0 += kBatch) {
     uint32_t j[kBatch];
 #pragma unroll
-    for (int b = 0; b < kBatch; ++b)
-      j[b] = (k0 + b < c) ? (uint32_t)L[(size_t)(k0 + b) * kWave] : pad;
+    for (int b = 0; b < kBatch; ++b) j[b] = (k0 + b < c) ? L[(size_t)(k0 + b) * kWave] : self;
     T q[kBatch];
 #pragma unroll
     for (int b = 0; b < kBatch; ++b) q[b] = load(j[b]);
@@ -118,71 +114,21 @@ __device__ __forceinline__ void add_counter64(uint32_t *stats, int slot, uint32_
   if (v) atomicAdd(reinterpret_cast<unsigned long long *>(stats + slot), (unsigned long long)v);
 }
 
-// ---------------------------------------------------------------------------
-// Block scans over kTile threads
-// ---------------------------------------------------------------------------
-constexpr int kTileWaves = kTile / kWave;
-
-// Exclusive sum and exclusive max of NV values per thread across the block,
-// with one pair of barriers for all NV scans.  wsum/wmax: NV*kTileWaves words.
-template <int NV>
-__device__ __forceinline__ void block_excl_sum(uint32_t (&x)[NV], uint32_t (&total)[NV],
-                                               uint32_t *wbuf) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  uint32_t inc[NV];
+// Block sum of v written to partial[blockIdx.x] (no atomics; the host adds the
+// partials when it reads the statistics).  All threads of the block call it.
+__device__ __forceinline__ void block_sum_to(uint32_t *partial, uint32_t v) {
+  __shared__ uint32_t wsum[kBlock / kWave];
+  uint32_t t = v;
 #pragma unroll
-  for (int v = 0; v < NV; ++v) {
-    uint32_t s = x[v];
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const uint32_t y = __shfl_up(s, o, 64);
-      if (lane >= o) s += y;
-    }
-    inc[v] = s;
-    if (lane == 63) wbuf[v * kTileWaves + w] = s;
-  }
+  for (int o = 32; o >= 1; o >>= 1) t += __shfl_xor(t, o, 64);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = t;
   __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t s = 0;
 #pragma unroll
-  for (int v = 0; v < NV; ++v) {
-    uint32_t pre = 0, all = 0;
-#pragma unroll
-    for (int i = 0; i < kTileWaves; ++i) {
-      const uint32_t t = wbuf[v * kTileWaves + i];
-      pre += (i < w) ? t : 0u;
-      all += t;
-    }
-    total[v] = all;
-    x[v] = pre + inc[v] - x[v];
+    for (int w = 0; w < kBlock / kWave; ++w) s += wsum[w];
+    partial[blockIdx.x] = s;
   }
-  __syncthreads();
-}
-
-template <int NV>
-__device__ __forceinline__ void block_excl_max(uint32_t (&x)[NV], uint32_t *wbuf) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  uint32_t exc[NV];
-#pragma unroll
-  for (int v = 0; v < NV; ++v) {
-    uint32_t s = x[v];
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const uint32_t y = __shfl_up(s, o, 64);
-      if (lane >= o) s = max(s, y);
-    }
-    const uint32_t e = __shfl_up(s, 1, 64);
-    exc[v] = lane ? e : 0u;
-    if (lane == 63) wbuf[v * kTileWaves + w] = s;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int v = 0; v < NV; ++v) {
-    uint32_t pre = 0;
-#pragma unroll
-    for (int i = 0; i < kTileWaves; ++i)
-      if (i < w) pre = max(pre, wbuf[v * kTileWaves + i]);
-    x[v] = max(pre, exc[v]);
-  }
-  __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
@@ -277,49 +223,34 @@ __device__ __noinline__ bool inside_exact(float px, float py, float pz, float qx
   return r2 < h2d;
 }
 
-__device__ __forceinline__ bool is_neighbor(const float4 &p, const float4 &q, const DevParams &P) {
-  const float ddx = p.x - q.x, ddy = p.y - q.y, ddz = p.z - q.z;
-  const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-  if (r2 < P.h2_lo) return true;
-  if (r2 >= P.h2_hi) return false;
-  return inside_exact(p.x, p.y, p.z, q.x, q.y, q.z, P.h2d);
-}
+// One thread per sorted particle.  The 3 cells (dx = -1..1) of each (dz, dy)
+// row are consecutive keys, so each row is one contiguous range of sorted
+// indices; walking rows in (dz, dy) order and indices upward reproduces the
+// reference's (dz, dy, dx, id) order and therefore its 60-cap truncation.
+//
+// Hits are buffered in LDS as 16 bits (dz layer, offset from the layer's
+// first index) and written out after the scan: a wave then stores one
+// 256-byte row of the slot-major list per instruction.  Written as found,
+// lanes store their k-th neighbour at different times and partially written
+// lines are written back several times.  A particle whose dz layer spans
+// more than kLayerSpan indices (never near rest density: a layer holds about
+// two grid rows of particles) writes its hits directly.
+constexpr uint32_t kLayerSpan = 1u << 14;
 
-// One workgroup per tile, one thread per sorted particle.  The 3 cells
-// (dx = -1..1) of each (dz, dy) row have consecutive keys, so each row is one
-// contiguous range [a_r, b_r) of sorted indices; walking rows in (dz, dy)
-// order and indices upward reproduces the reference's (dz, dy, dx, id) order
-// and therefore its 60-cap truncation.  For a fixed row offset r, a_r and b_r
-// are non-decreasing along the sorted tile, so the union of the tile's ranges
-// for r is found with one segmented scan.
-__global__ __launch_bounds__(kTile) void k_build_lists(
-    uint32_t n, uint32_t ntiles, const float4 *__restrict__ pos,
+__global__ __launch_bounds__(kBlock) void k_build_lists(
+    uint32_t n, uint32_t nblocks, const float4 *__restrict__ pos,
     const uint32_t *__restrict__ key_s, const uint32_t *__restrict__ cstart, DevParams P,
-    uint16_t *__restrict__ list16, uint32_t *__restrict__ list32, int *__restrict__ cnt,
-    int2 *__restrict__ tile_meta, uint2 *__restrict__ tile_int, uint32_t *__restrict__ stats,
+    uint32_t *__restrict__ list, int *__restrict__ cnt, uint32_t *__restrict__ stats,
     uint32_t *__restrict__ pair_partial) {
-  extern __shared__ float4 spos[];  // kCap staged x*
-  __shared__ uint16_t sout[kMaxNb][kTile];
-  __shared__ uint32_t seg_a[9][kMaxSeg], seg_b[9][kMaxSeg], seg_off[9][kMaxSeg];
-  __shared__ uint32_t int_start[kMaxInt], int_off[kMaxInt];
-  __shared__ uint32_t nseg[9];
-  __shared__ uint32_t wbuf[9 * kTileWaves];
-  __shared__ int s_nint;
-  __shared__ uint32_t s_total;
-  const uint32_t tid = threadIdx.x;
-  const uint32_t t = xcd_block(blockIdx.x, ntiles);
-  const uint32_t s = t * kTile + tid;
-  const bool active = s < n;
-
-  for (uint32_t i = tid; i < 9 * kMaxSeg; i += kTile) (&seg_b[0][0])[i] = 0u;
-
-  // row ranges of this particle
-  float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  uint32_t a[9], b[9];
-#pragma unroll
-  for (int r = 0; r < 9; ++r) a[r] = b[r] = 0u;
-  if (active) {
-    p = pos[s];
+  __shared__ uint16_t sbuf[kMaxNb][kBlock];
+  const uint32_t s = particle_index(nblocks);
+  int c = 0;
+  uint32_t dropped = 0;
+  bool direct = false;
+  uint32_t base[3] = {0u, 0u, 0u};
+  uint32_t *out = list + (size_t)(s >> 6) * kListStride + (s & 63u);
+  if (s < n) {
+    const float4 p = pos[s];
     const uint32_t key = key_s[s];
     const uint32_t cz = key / P.nxy;
     const uint32_t rem = key - cz * P.nxy;
@@ -327,218 +258,83 @@ __global__ __launch_bounds__(kTile) void k_build_lists(
     const uint32_t cx = rem - cy * P.nx;
     const uint32_t x0 = cx > 0 ? cx - 1 : 0;
     const uint32_t x1 = cx + 1 < P.nx ? cx + 1 : P.nx - 1;
+    const uint32_t y0 = cy > 0 ? cy - 1 : 0;
+    const uint32_t y1 = cy + 1 < P.ny ? cy + 1 : P.ny - 1;
 #pragma unroll
-    for (int r = 0; r < 9; ++r) {
-      const int z = (int)cz + r / 3 - 1, y = (int)cy + r % 3 - 1;
-      if (z < 0 || z >= (int)P.nz || y < 0 || y >= (int)P.ny) continue;
-      const uint32_t row = (uint32_t)z * P.nxy + (uint32_t)y * P.nx;
-      a[r] = cstart[row + x0];
-      b[r] = cstart[row + x1 + 1];
+    for (int l = 0; l < 3; ++l) {
+      const int z = (int)cz + l - 1;
+      if (z < 0 || z >= (int)P.nz) continue;
+      const uint32_t zrow = (uint32_t)z * P.nxy;
+      base[l] = cstart[zrow + y0 * P.nx + x0];
+      if (cstart[zrow + y1 * P.nx + x1 + 1] - base[l] > kLayerSpan) direct = true;
     }
-  }
-
-  // staging map: per r, merge the tile's ranges into segments
-  uint32_t prevb[9], nprev[9], starts[9], tot[9];
+    for (int dz = -1; dz <= 1; ++dz) {
+      const int z = (int)cz + dz;
+      if (z < 0 || z >= (int)P.nz) continue;
+      const uint32_t lb = dz < 0 ? base[0] : (dz == 0 ? base[1] : base[2]);
+      const uint32_t tag = (uint32_t)(dz + 1) << 14;
+      for (int dy = -1; dy <= 1; ++dy) {
+        const int y = (int)cy + dy;
+        if (y < 0 || y >= (int)P.ny) continue;
+        const uint32_t row = (uint32_t)z * P.nxy + (uint32_t)y * P.nx;
+        const uint32_t j0 = cstart[row + x0];
+        const uint32_t j1 = cstart[row + x1 + 1];
+        // candidates in batches of kBatch loads; hits are appended in j order
+        for (uint32_t jb = j0; jb < j1; jb += kBatch) {
+          float4 q[kBatch];
 #pragma unroll
-  for (int r = 0; r < 9; ++r) {
-    const bool v = a[r] < b[r];
-    prevb[r] = v ? b[r] : 0u;
-    nprev[r] = v ? 1u : 0u;
-  }
-  block_excl_max<9>(prevb, wbuf);
-  block_excl_sum<9>(nprev, tot, wbuf);
+          for (int b = 0; b < kBatch; ++b) q[b] = pos[(jb + b < j1) ? jb + b : s];
 #pragma unroll
-  for (int r = 0; r < 9; ++r) {
-    const bool v = a[r] < b[r];
-    starts[r] = (v && (nprev[r] == 0u || a[r] > prevb[r])) ? 1u : 0u;
-  }
-  uint32_t seg[9];
-#pragma unroll
-  for (int r = 0; r < 9; ++r) seg[r] = starts[r];
-  block_excl_sum<9>(seg, tot, wbuf);  // seg = index of the segment this range starts/joins
-#pragma unroll
-  for (int r = 0; r < 9; ++r) {
-    const bool v = a[r] < b[r];
-    if (v && !starts[r]) seg[r] -= 1u;  // joined the segment before it
-    if (starts[r] && seg[r] < kMaxSeg) seg_a[r][seg[r]] = a[r];
-    if (v && seg[r] < kMaxSeg) atomicMax(&seg_b[r][seg[r]], b[r]);
-    if (tid == 0) nseg[r] = tot[r];
-  }
-  __syncthreads();
-  if (tid == 0) {
-    uint32_t off = 0;
-    int ni = 0;
-    bool ok = true;
-    for (int r = 0; r < 9 && ok; ++r) {
-      if (nseg[r] > (uint32_t)kMaxSeg) {
-        ok = false;
-        break;
-      }
-      for (uint32_t i = 0; i < nseg[r]; ++i) {
-        if (ni >= kMaxInt) {
-          ok = false;
-          break;
-        }
-        seg_off[r][i] = off;
-        int_start[ni] = seg_a[r][i];
-        int_off[ni] = off;
-        off += seg_b[r][i] - seg_a[r][i];
-        ++ni;
-      }
-    }
-    if (off > (uint32_t)kCap) ok = false;
-    s_nint = ok ? ni : -1;
-    s_total = off;
-  }
-  __syncthreads();
-  const int nint = s_nint;
-  const bool staged = nint >= 0;
-  const uint32_t total = staged ? s_total : 0u;
-  if (tid == 0) {
-    tile_meta[t] = make_int2(nint, (int)total);
-    if (!staged) atomicAdd(&stats[STAT_GLOBAL_TILES], 1u);
-  }
-  if (staged && tid < (uint32_t)nint) tile_int[(size_t)t * kMaxInt + tid] = make_uint2(int_start[tid], int_off[tid]);
-  if (staged) {
-    for (uint32_t i = tid; i < total; i += kTile) {
-      int lo = 0, hi = nint - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (int_off[mid] <= i) lo = mid; else hi = mid - 1;
-      }
-      spos[i] = pos[int_start[lo] + (i - int_off[lo])];
-    }
-    __syncthreads();
-  }
-
-  // candidates, in batches of kBatch loads; hits appended in (r, j) order
-  int c = 0;
-  uint32_t dropped = 0;
-  uint32_t *out32 = list32 + (size_t)(s >> 6) * kListStride + (s & 63u);
-#pragma unroll 1
-  for (int r = 0; r < 9; ++r) {
-    if (!(a[r] < b[r])) continue;
-    const uint32_t bias = staged ? seg_off[r][seg[r]] - seg_a[r][seg[r]] : 0u;
-    for (uint32_t jb = a[r]; jb < b[r]; jb += kBatch) {
-      float4 q[kBatch];
-#pragma unroll
-      for (int k = 0; k < kBatch; ++k) {
-        const uint32_t j = (jb + k < b[r]) ? jb + k : a[r];
-        q[k] = staged ? spos[j + bias] : pos[j];
-      }
-#pragma unroll
-      for (int k = 0; k < kBatch; ++k) {
-        const uint32_t j = jb + k;
-        if (j >= b[r] || j == s) continue;
-        if (is_neighbor(p, q[k], P)) {
-          if (c < kMaxNb) {
-            if (staged)
-              sout[c][tid] = (uint16_t)(j + bias);
+          for (int b = 0; b < kBatch; ++b) {
+            const uint32_t j = jb + b;
+            if (j >= j1 || j == s) continue;
+            const float ddx = p.x - q[b].x, ddy = p.y - q[b].y, ddz = p.z - q[b].z;
+            const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+            bool hit;
+            if (r2 < P.h2_lo)
+              hit = true;
+            else if (r2 >= P.h2_hi)
+              hit = false;
             else
-              out32[(size_t)c * kWave] = j;
-            ++c;
-          } else {
-            ++dropped;
+              hit = inside_exact(p.x, p.y, p.z, q[b].x, q[b].y, q[b].z, P.h2d);
+            if (hit) {
+              if (c < kMaxNb) {
+                if (direct)
+                  out[(size_t)c * kWave] = j;
+                else
+                  sbuf[c][threadIdx.x] = (uint16_t)(tag | (j - lb));
+                ++c;
+              } else {
+                ++dropped;
+              }
+            }
           }
         }
       }
     }
+    cnt[s] = c;
   }
-  if (active) cnt[s] = c;
+  // coalesced write-out: every lane of the wave walks the wave's longest list
+  int cmax = direct ? 0 : c;
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) cmax = max(cmax, __shfl_xor(cmax, o, 64));
+  for (int k = 0; k < cmax; ++k) {
+    if (k < c && !direct) {
+      const uint32_t e = sbuf[k][threadIdx.x];
+      const uint32_t l = e >> 14;
+      const uint32_t b = l == 0 ? base[0] : (l == 1 ? base[1] : base[2]);
+      out[(size_t)k * kWave] = b + (e & (kLayerSpan - 1u));
+    }
+  }
   add_counter64(stats, STAT_DROPPED, dropped);
-  // tile sum (statistics) and tile max (rows to write) of the list lengths
-  uint32_t csum = (uint32_t)c, cmax = (uint32_t)c;
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) {
-    csum += __shfl_xor(csum, o, 64);
-    cmax = max(cmax, (uint32_t)__shfl_xor(cmax, o, 64));
-  }
-  __syncthreads();  // wbuf reuse; sout complete
-  if ((tid & 63) == 0) {
-    wbuf[tid >> 6] = csum;
-    wbuf[kTileWaves + (tid >> 6)] = cmax;
-  }
-  __syncthreads();
-  uint32_t tile_sum = 0, tile_max = 0;
-#pragma unroll
-  for (int w = 0; w < kTileWaves; ++w) {
-    tile_sum += wbuf[w];
-    tile_max = max(tile_max, wbuf[kTileWaves + w]);
-  }
-  if (tid == 0) pair_partial[t] = tile_sum;
-  if (staged && active) {
-    // the tile's lists row by row: each wave writes 64 consecutive slots
-    uint16_t *out16 = list16 + (size_t)(s >> 6) * kListStride + (s & 63u);
-    for (uint32_t k = 0; k < tile_max; ++k) out16[(size_t)k * kWave] = sout[k][tid];
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Tile staging for the gather passes
-// ---------------------------------------------------------------------------
-struct TileLists {
-  const uint16_t *l16;
-  const uint32_t *l32;
-  const int *cnt;
-  const int2 *meta;
-  const uint2 *ints;
-  uint32_t ntiles;
-};
-
-// Copies load(j) for every index j of the tile's staging map into lds (in
-// map order).  Returns false (and stages nothing) for a global-mode tile.
-// Uniform across the workgroup; all threads must call it.
-template <typename T, typename Load>
-__device__ __forceinline__ bool stage_tile(uint32_t t, const TileLists &TL, T *lds, Load load) {
-  __shared__ uint32_t ist[kMaxInt], ioff[kMaxInt];
-  const int2 meta = TL.meta[t];
-  if (meta.x < 0) return false;
-  const int nint = meta.x;
-  const uint32_t total = (uint32_t)meta.y;
-  if (threadIdx.x < (uint32_t)nint) {
-    const uint2 iv = TL.ints[(size_t)t * kMaxInt + threadIdx.x];
-    ist[threadIdx.x] = iv.x;
-    ioff[threadIdx.x] = iv.y;
-  }
-  __syncthreads();
-  constexpr int U = 4;
-  for (uint32_t i0 = threadIdx.x; i0 < total; i0 += U * kTile) {
-    T v[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const uint32_t i = i0 + u * kTile;
-      if (i < total) {
-        int lo = 0, hi = nint - 1;
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (ioff[mid] <= i) lo = mid; else hi = mid - 1;
-        }
-        v[u] = load(ist[lo] + (i - ioff[lo]));
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const uint32_t i = i0 + u * kTile;
-      if (i < total) lds[i] = v[u];
-    }
-  }
-  __syncthreads();
-  return true;
-}
-
-// Neighbour visit from LDS (staged tile) or from global memory.
-template <typename T, typename Load, typename Body>
-__device__ __forceinline__ void visit(bool staged, const TileLists &TL, uint32_t s, int c,
-                                      const T *lds, Load gload, Body body) {
-  if (staged)
-    for_neighbors<T>(list_ptr(TL.l16, s), c, 0u, [&](uint32_t k) { return lds[k]; }, body);
-  else
-    for_neighbors<T>(list_ptr(TL.l32, s), c, s, gload, body);
+  block_sum_to(pair_partial, (uint32_t)c);
 }
 
 // ---------------------------------------------------------------------------
 // Jacobi constraint loop
 // ---------------------------------------------------------------------------
+// Max of a non-negative float over the wave, then one atomic per wave on the
+// float's bit pattern (monotonic for non-negative floats).
 // The slot only grows during a launch, so a wave whose maximum does not exceed
 // the value it reads skips the atomic; without this every wave would issue a
 // device-scope atomic to one address.
@@ -561,23 +357,20 @@ __device__ __forceinline__ bool converged(const uint32_t *stats) {
 // spiky gradients.  Writes rho and pos.w = lambda.
 // MODE 0: both, 1: density only (rho), 2: lambda only (reads rho).
 template <int MODE>
-__global__ __launch_bounds__(kTile) void k_density_lambda(uint32_t n, TileLists TL, float4 *pos,
-                                                          DevParams P, float *rho_io,
-                                                          uint32_t *stats, int err_slot,
-                                                          int check_converged,
-                                                          int density_to_w) {
-  extern __shared__ float4 lds4[];
+__global__ __launch_bounds__(kBlock) void k_density_lambda(
+    uint32_t n, uint32_t nblocks, float4 *pos, const uint32_t *__restrict__ list,
+    const int *__restrict__ cnt, DevParams P, float *rho_io, uint32_t *stats, int err_slot,
+    int check_converged, int density_to_w) {
   if (check_converged && converged(stats)) return;
-  const uint32_t t = xcd_block(blockIdx.x, TL.ntiles);
-  const uint32_t s = t * kTile + threadIdx.x;
-  const bool staged = stage_tile(t, TL, lds4, [&](uint32_t j) { return pos[j]; });
+  const uint32_t s = particle_index(nblocks);
   const bool active = s < n;
   float err = 0.0f;
   if (active) {
     const float4 p = pos[s];
-    const int c = TL.cnt[s];
+    const int c = cnt[s];
+    const uint32_t *L = list_ptr(list, s);
     float su3 = 0.0f, sgx = 0.0f, sgy = 0.0f, sgz = 0.0f, sc = 0.0f;
-    visit(staged, TL, s, c, lds4, [&](uint32_t j) { return pos[j]; }, [&](const float4 &q) {
+    for_neighbors<float4>(L, c, s, [&](uint32_t j) { return pos[j]; }, [&](const float4 &q) {
       const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
       const float r2 = dx * dx + dy * dy + dz * dz;
       if (MODE != 2) {
@@ -662,11 +455,11 @@ __device__ __forceinline__ void collide(float &x, float &y, float &z, const SdfD
   z += dist * (gz * inv);
 }
 
-// UpdateDPs (particles.c:668-746): sum_j (lambda_i + lambda_j) DelW(r)/r d.
-template <typename Visit>
-__device__ __forceinline__ float3 dp_sum(const float4 &p, Visit v, const DevParams &P) {
+// UpdateDPs (particles.c:668-746): returns sum_j (lambda_i + lambda_j) DelW(r)/r d.
+__device__ __forceinline__ float3 dp_sum(const float4 &p, const float4 *pos, const uint32_t *L,
+                                         int c, uint32_t self, const DevParams &P) {
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
-  v([&](const float4 &q) {
+  for_neighbors<float4>(L, c, self, [&](uint32_t j) { return pos[j]; }, [&](const float4 &q) {
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     const float r2 = dx * dx + dy * dy + dz * dz;
     const float r2c = fmaxf(r2, P.rmin2);
@@ -682,18 +475,15 @@ __device__ __forceinline__ float3 dp_sum(const float4 &p, Visit v, const DevPara
 }
 
 // Staged UpdateDPs: dp = stiffness * m * DelW_norm / rho0 * sum.
-__global__ __launch_bounds__(kTile) void k_dp(uint32_t n, TileLists TL,
-                                              const float4 *__restrict__ pos, DevParams P,
-                                              float scale, float4 *__restrict__ dp) {
-  extern __shared__ float4 lds4[];
-  const uint32_t t = xcd_block(blockIdx.x, TL.ntiles);
-  const uint32_t s = t * kTile + threadIdx.x;
-  auto gload = [&](uint32_t j) { return pos[j]; };
-  const bool staged = stage_tile(t, TL, lds4, gload);
+__global__ __launch_bounds__(kBlock) void k_dp(uint32_t n, uint32_t nblocks,
+                                               const float4 *__restrict__ pos,
+                                               const uint32_t *__restrict__ list,
+                                               const int *__restrict__ cnt, DevParams P,
+                                               float scale, float4 *__restrict__ dp) {
+  const uint32_t s = particle_index(nblocks);
   if (s >= n) return;
   const float4 p = pos[s];
-  const int c = TL.cnt[s];
-  const float3 a = dp_sum(p, [&](auto body) { visit(staged, TL, s, c, lds4, gload, body); }, P);
+  const float3 a = dp_sum(p, pos, list_ptr(list, s), cnt[s], s, P);
   dp[s] = make_float4(scale * a.x, scale * a.y, scale * a.z, 0.0f);
 }
 
@@ -716,35 +506,26 @@ __global__ __launch_bounds__(kBlock) void k_position_stars(uint32_t n, float4 *_
 // (x*', rho).  In tolerance mode, once the density pass of this iteration is
 // within tolerance, the particles are copied unchanged instead.  With lam_acc,
 // adds stiff * lambda_i to the particle's accumulated multiplier.
-__global__ __launch_bounds__(kTile) void k_dp_update(uint32_t n, TileLists TL,
-                                                     const float4 *__restrict__ pin,
-                                                     float4 *__restrict__ pout,
-                                                     const float *__restrict__ rho, DevParams P,
-                                                     float scale, SdfDev o, uint32_t *stats,
-                                                     int err_slot, float tol,
-                                                     float *__restrict__ lam_acc, float stiff) {
-  extern __shared__ float4 lds4[];
+__global__ __launch_bounds__(kBlock) void k_dp_update(
+    uint32_t n, uint32_t nblocks, const float4 *__restrict__ pin, float4 *__restrict__ pout,
+    const float *__restrict__ rho, const uint32_t *__restrict__ list,
+    const int *__restrict__ cnt, DevParams P, float scale, SdfDev o, uint32_t *stats,
+    int err_slot, float tol, float *__restrict__ lam_acc, float stiff) {
   bool done = false;
   if (tol > 0.0f) {
-    done = converged(stats) || __uint_as_float(stats[STAT_ERR0 + err_slot]) <= tol;
+    done = converged(stats) ||
+           __uint_as_float(stats[STAT_ERR0 + err_slot]) <= tol;
   }
-  const uint32_t t = xcd_block(blockIdx.x, TL.ntiles);
-  const uint32_t s = t * kTile + threadIdx.x;
+  const uint32_t s = particle_index(nblocks);
+  if (s >= n) return;
+  const float4 p = pin[s];
   if (done) {
-    if (s < n) {
-      const float4 p = pin[s];
-      if (s == 0) stats[STAT_CONVERGED] = 1u;
-      pout[s] = make_float4(p.x, p.y, p.z, rho[s]);
-    }
+    if (s == 0) stats[STAT_CONVERGED] = 1u;
+    pout[s] = make_float4(p.x, p.y, p.z, rho[s]);
     return;
   }
-  auto gload = [&](uint32_t j) { return pin[j]; };
-  const bool staged = stage_tile(t, TL, lds4, gload);
-  if (s >= n) return;
   if (s == 0) stats[STAT_ITERS] += 1u;
-  const float4 p = pin[s];
-  const int c = TL.cnt[s];
-  const float3 a = dp_sum(p, [&](auto body) { visit(staged, TL, s, c, lds4, gload, body); }, P);
+  const float3 a = dp_sum(p, pin, list_ptr(list, s), cnt[s], s, P);
   float x = p.x + scale * a.x, y = p.y + scale * a.y, z = p.z + scale * a.z;
   apply_boundary(x, y, z, P);
   collide(x, y, z, o);
@@ -783,34 +564,33 @@ __device__ __forceinline__ void sdf_features(float x, float y, float z, const Sd
 
 // First pass of a warm-started or logged solve.  ComputeDensities and
 // ComputeLambda as in k_density_lambda, plus the network features, in one
-// gather over LDS-staged (x*, x) pairs.  LOG: writes the feature rows and the
+// gather of (x*, x) per neighbour.  LOG: writes the feature rows and the
 // plain lambda.  Otherwise runs the network on MFMA (BF16, FP32 accumulate)
 // and writes Lambda_pred = -max(C_eff, 0) / (|grad C|^2 + eps) as lambda.
-// Either way the particle's accumulated multiplier restarts at zero.
+// Either way the particle's accumulated multiplier restarts at zero.  Every
+// lane of a wave reaches the network (no early return).
 template <bool LOG>
-__global__ __launch_bounds__(kTile) void k_warm_lambda(
-    uint32_t n, TileLists TL, float4 *pos, const float4 *__restrict__ xold, DevParams P,
+__global__ __launch_bounds__(kBlock) void k_warm_lambda(
+    uint32_t n, uint32_t nblocks, float4 *pos, const float4 *__restrict__ xold,
+    const uint32_t *__restrict__ list, const int *__restrict__ cnt, DevParams P,
     float *__restrict__ rho_io, uint32_t *stats, const uint32_t *__restrict__ id_s,
     const float *__restrict__ lam_prev, SdfDev o, const MlpPacked *__restrict__ net,
-    float *__restrict__ lam_acc, float *__restrict__ feat) {
-  extern __shared__ Pair4 lds8[];
-  const uint32_t t = xcd_block(blockIdx.x, TL.ntiles);
-  const uint32_t s = t * kTile + threadIdx.x;
-  auto gload = [&](uint32_t j) { return Pair4{pos[j], xold[j]}; };
-  const bool staged = stage_tile(t, TL, lds8, gload);
+    float *__restrict__ lam_acc, float *__restrict__ feat, float2 *__restrict__ plain) {
+  const uint32_t s = particle_index(nblocks);
   const bool active = s < n;
   float f[kMlpPad];
 #pragma unroll
   for (int i = 0; i < kMlpPad; ++i) f[i] = 0.0f;
-  float err = 0.0f, lam0 = 0.0f, denom = 1.0f;
+  float err = 0.0f, lam0 = 0.0f, denom = 1.0f, rho = 0.0f;
   if (active) {
     const float4 p = pos[s];
     const float4 po = xold[s];
     const float ux = p.x - po.x, uy = p.y - po.y, uz = p.z - po.z;  // dt * v_i
-    const int c = TL.cnt[s];
+    const int c = cnt[s];
     float su3 = 0.0f, sgx = 0.0f, sgy = 0.0f, sgz = 0.0f, sc = 0.0f;
     float mx = 0.0f, my = 0.0f, mz = 0.0f, dv = 0.0f;
-    visit(staged, TL, s, c, lds8, gload, [&](const Pair4 &qq) {
+    auto load = [&](uint32_t j) { return Pair4{pos[j], xold[j]}; };
+    for_neighbors<Pair4>(list_ptr(list, s), c, s, load, [&](const Pair4 &qq) {
       const float4 &q = qq.a, &qo = qq.b;
       const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
       const float r2 = dx * dx + dy * dy + dz * dz;
@@ -831,7 +611,7 @@ __global__ __launch_bounds__(kTile) void k_warm_lambda(
       const float ex = (q.x - qo.x) - ux, ey = (q.y - qo.y) - uy, ez = (q.z - qo.z) - uz;
       dv += g * (ex * dx + ey * dy + ez * dz);
     });
-    const float rho = P.mW0 + P.mWn * su3;
+    rho = P.mW0 + P.mWn * su3;
     rho_io[s] = rho;
     const float cr = (rho - P.rest_density) * P.inv_rest_density;
     err = fmaxf(cr, 0.0f);
@@ -860,12 +640,43 @@ __global__ __launch_bounds__(kTile) void k_warm_lambda(
       row[3] = make_float4(denom, lam0, 0.0f, 0.0f);
     }
   } else {
-    const float y = mlp_wave(f, net);  // every lane of the wave takes part
-    lam = -fmaxf(y, 0.0f) / denom;
+    const float y = mlp_wave(f, net);
+    const float ymax = net->norm[kNormYMax];
+    const float ce = ymax > 0.0f ? fminf(fmaxf(y, 0.0f), ymax) : fmaxf(y, 0.0f);
+    lam = -ce / denom;
+    if (active) plain[s] = make_float2(lam0, rho);  // for the plain fallback
   }
   if (active) {
     pos[s].w = lam;
     lam_acc[s] = 0.0f;
+  }
+}
+
+// Plain fallback of a warm start, after the density pass of iteration 1: if
+// the predicted step left the maximum density error above the error before it,
+// the particles return to their positions before the step with the plain
+// first-iteration lambda and density, and iteration 1 proceeds from there
+// (its error slot then holds the pre-step error).
+__global__ __launch_bounds__(kBlock) void k_warm_fallback(uint32_t n, uint32_t *stats,
+                                                          const float4 *__restrict__ pre,
+                                                          float4 *__restrict__ pos,
+                                                          const float2 *__restrict__ plain,
+                                                          float *__restrict__ rho,
+                                                          float *__restrict__ lam_acc) {
+  if (stats[STAT_CONVERGED]) return;
+  const uint32_t e0 = stats[STAT_ERR0], e1 = stats[STAT_ERR0 + 1];
+  if (e1 <= e0) return;  // non-negative floats compare as their bit patterns
+  const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+  if (s < n) {
+    const float4 p = pre[s];
+    const float2 q = plain[s];
+    pos[s] = make_float4(p.x, p.y, p.z, q.x);
+    rho[s] = q.y;
+    lam_acc[s] = 0.0f;
+  }
+  if (s == 0) {
+    stats[STAT_FALLBACK] = 1u;
+    stats[STAT_ERR0 + 1] = e0;
   }
 }
 
@@ -923,22 +734,21 @@ __global__ __launch_bounds__(kBlock) void k_velocity(uint32_t n, const float4 *_
 // with UpdateVelocities (own particle only).  DelWPoly(r)/r = norm (h^2-r^2)^2
 // with r floored at 1e-4 h.
 template <bool WITH_VELOCITY>
-__global__ __launch_bounds__(kTile) void k_color(uint32_t n, TileLists TL,
-                                                 const float4 *__restrict__ pos,
-                                                 const float4 *__restrict__ xold, DevParams P,
-                                                 float4 *__restrict__ vel,
-                                                 float4 *__restrict__ color) {
-  extern __shared__ float4 lds4[];
-  const uint32_t t = xcd_block(blockIdx.x, TL.ntiles);
-  const uint32_t s = t * kTile + threadIdx.x;
-  auto gload = [&](uint32_t j) { return pos[j]; };
-  const bool staged = stage_tile(t, TL, lds4, gload);
+__global__ __launch_bounds__(kBlock) void k_color(uint32_t n, uint32_t nblocks,
+                                                  const float4 *__restrict__ pos,
+                                                  const float4 *__restrict__ xold,
+                                                  const uint32_t *__restrict__ list,
+                                                  const int *__restrict__ cnt, DevParams P,
+                                                  float4 *__restrict__ vel,
+                                                  float4 *__restrict__ color) {
+  const uint32_t s = particle_index(nblocks);
   if (s >= n) return;
   const float4 p = pos[s];
   if (WITH_VELOCITY) vel[s] = velocity_of(p, xold[s], P);
-  const int c = TL.cnt[s];
+  const uint32_t *L = list_ptr(list, s);
+  const int c = cnt[s];
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
-  visit(staged, TL, s, c, lds4, gload, [&](const float4 &q) {
+  for_neighbors<float4>(L, c, s, [&](uint32_t j) { return pos[j]; }, [&](const float4 &q) {
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     const float r2c = fmaxf(dx * dx + dy * dy + dz * dz, P.rmin2);
     const float u = P.h2 - r2c;
@@ -952,22 +762,22 @@ __global__ __launch_bounds__(kTile) void k_color(uint32_t n, TileLists TL,
 
 // ComputeSurfaceTension force pass (particles.c:142-204): cohesion (Akinci
 // spline C, particles.c:54-65) + curvature, v += dt F / rho_i in place.
-__global__ __launch_bounds__(kTile) void k_surface(uint32_t n, TileLists TL,
-                                                   const float4 *__restrict__ pos,
-                                                   const float4 *__restrict__ color, DevParams P,
-                                                   float4 *__restrict__ vel) {
-  extern __shared__ Pair4 lds8[];
-  const uint32_t t = xcd_block(blockIdx.x, TL.ntiles);
-  const uint32_t s = t * kTile + threadIdx.x;
-  auto gload = [&](uint32_t j) { return Pair4{pos[j], color[j]}; };
-  const bool staged = stage_tile(t, TL, lds8, gload);
+__global__ __launch_bounds__(kBlock) void k_surface(uint32_t n, uint32_t nblocks,
+                                                    const float4 *__restrict__ pos,
+                                                    const float4 *__restrict__ color,
+                                                    const uint32_t *__restrict__ list,
+                                                    const int *__restrict__ cnt, DevParams P,
+                                                    float4 *__restrict__ vel) {
+  const uint32_t s = particle_index(nblocks);
   if (s >= n) return;
   const float4 p = pos[s];
   const float4 ci = color[s];
-  const int c = TL.cnt[s];
+  const uint32_t *L = list_ptr(list, s);
+  const int c = cnt[s];
   const float hh = 0.5f * P.h;
   float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-  visit(staged, TL, s, c, lds8, gload, [&](const Pair4 &qc) {
+  auto load = [&](uint32_t j) { return Pair4{pos[j], color[j]}; };
+  for_neighbors<Pair4>(L, c, s, load, [&](const Pair4 &qc) {
     const float4 &q = qc.a, &cq = qc.b;
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     const float r2c = fmaxf(dx * dx + dy * dy + dz * dz, P.rmin2);
@@ -975,8 +785,8 @@ __global__ __launch_bounds__(kTile) void k_surface(uint32_t n, TileLists TL,
     const float r = r2c * ir;
     float spline = 0.0f;
     if (r <= P.h) {
-      const float tt = (P.h - r) * r;
-      const float t3 = tt * tt * tt;
+      const float t = (P.h - r) * r;
+      const float t3 = t * t * t;
       spline = r <= hh ? 2.0f * t3 - P.h6_64 : t3;
     }
     const float coh = P.coh_k * spline * ir;
@@ -995,21 +805,21 @@ __global__ __launch_bounds__(kTile) void k_surface(uint32_t n, TileLists TL,
 
 // ApplyViscosity (particles.c:383-459), reading v_old and writing v_new.  The
 // distance keeps the reference's typo r^2 = dx^2 + dy^2 dz^2 (particles.c:439).
-__global__ __launch_bounds__(kTile) void k_xsph(uint32_t n, TileLists TL,
-                                                const float4 *__restrict__ pos,
-                                                const float4 *__restrict__ vin, DevParams P,
-                                                float4 *__restrict__ vout) {
-  extern __shared__ Pair4 lds8[];
-  const uint32_t t = xcd_block(blockIdx.x, TL.ntiles);
-  const uint32_t s = t * kTile + threadIdx.x;
-  auto gload = [&](uint32_t j) { return Pair4{pos[j], vin[j]}; };
-  const bool staged = stage_tile(t, TL, lds8, gload);
+__global__ __launch_bounds__(kBlock) void k_xsph(uint32_t n, uint32_t nblocks,
+                                                 const float4 *__restrict__ pos,
+                                                 const float4 *__restrict__ vin,
+                                                 const uint32_t *__restrict__ list,
+                                                 const int *__restrict__ cnt, DevParams P,
+                                                 float4 *__restrict__ vout) {
+  const uint32_t s = particle_index(nblocks);
   if (s >= n) return;
   const float4 p = pos[s];
   const float4 vi = vin[s];
-  const int c = TL.cnt[s];
+  const uint32_t *L = list_ptr(list, s);
+  const int c = cnt[s];
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
-  visit(staged, TL, s, c, lds8, gload, [&](const Pair4 &qv) {
+  auto load = [&](uint32_t j) { return Pair4{pos[j], vin[j]}; };
+  for_neighbors<Pair4>(L, c, s, load, [&](const Pair4 &qv) {
     const float4 &q = qv.a, &vq = qv.b;
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     const float u = P.h2 - (dx * dx + dy * dy * dz * dz);
@@ -1022,22 +832,24 @@ __global__ __launch_bounds__(kTile) void k_xsph(uint32_t n, TileLists TL,
   vout[s] = make_float4(vi.x + k * ax, vi.y + k * ay, vi.z + k * az, 0.0f);
 }
 
-// ComputeVorticity (particles.c:207-283); aux = (omega, |omega|).
-__global__ __launch_bounds__(kTile) void k_vorticity(uint32_t n, TileLists TL,
-                                                     const float4 *__restrict__ pos,
-                                                     const float4 *__restrict__ vel, DevParams P,
-                                                     float4 *__restrict__ omega) {
-  extern __shared__ Pair4 lds8[];
-  const uint32_t t = xcd_block(blockIdx.x, TL.ntiles);
-  const uint32_t s = t * kTile + threadIdx.x;
-  auto gload = [&](uint32_t j) { return Pair4{pos[j], vel[j]}; };
-  const bool staged = stage_tile(t, TL, lds8, gload);
+// ComputeVorticity (particles.c:207-283); aux = (omega, |omega|), and
+// pw = (x*, |omega|) so that the confinement gather is one float4 per neighbour.
+__global__ __launch_bounds__(kBlock) void k_vorticity(uint32_t n, uint32_t nblocks,
+                                                      const float4 *__restrict__ pos,
+                                                      const float4 *__restrict__ vel,
+                                                      const uint32_t *__restrict__ list,
+                                                      const int *__restrict__ cnt, DevParams P,
+                                                      float4 *__restrict__ omega,
+                                                      float4 *__restrict__ pw) {
+  const uint32_t s = particle_index(nblocks);
   if (s >= n) return;
   const float4 p = pos[s];
   const float4 vi = vel[s];
-  const int c = TL.cnt[s];
+  const uint32_t *L = list_ptr(list, s);
+  const int c = cnt[s];
   float ox = 0.0f, oy = 0.0f, oz = 0.0f;
-  visit(staged, TL, s, c, lds8, gload, [&](const Pair4 &qv) {
+  auto load = [&](uint32_t j) { return Pair4{pos[j], vel[j]}; };
+  for_neighbors<Pair4>(L, c, s, load, [&](const Pair4 &qv) {
     const float4 &q = qv.a, &vq = qv.b;
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     const float r2c = fmaxf(dx * dx + dy * dy + dz * dz, P.rmin2);
@@ -1055,7 +867,9 @@ __global__ __launch_bounds__(kTile) void k_vorticity(uint32_t n, TileLists TL,
   ox *= k;
   oy *= k;
   oz *= k;
-  omega[s] = make_float4(ox, oy, oz, sqrtf(ox * ox + oy * oy + oz * oz));
+  const float wn = sqrtf(ox * ox + oy * oy + oz * oz);
+  omega[s] = make_float4(ox, oy, oz, wn);
+  pw[s] = make_float4(p.x, p.y, p.z, wn);
 }
 
 // ApplyVorticityConfinement (particles.c:285-381): eta = sum |omega_j| gradW,
@@ -1063,24 +877,20 @@ __global__ __launch_bounds__(kTile) void k_vorticity(uint32_t n, TileLists TL,
 // is set the result is also UpdatePositions (particles.c:566-586): x* and v
 // are scattered back to particle-id order.
 template <bool COMMIT>
-__global__ __launch_bounds__(kTile) void k_confine(
-    uint32_t n, TileLists TL, const float4 *__restrict__ pos, const float4 *__restrict__ omega,
-    DevParams P, const float4 *__restrict__ vin, float4 *__restrict__ vout,
-    const uint32_t *__restrict__ id_s, float4 *__restrict__ x_can, float4 *__restrict__ v_can) {
-  extern __shared__ float4 lds4[];
-  const uint32_t t = xcd_block(blockIdx.x, TL.ntiles);
-  const uint32_t s = t * kTile + threadIdx.x;
-  // only |omega_j| (omega.w) of the neighbour is needed
-  auto gload = [&](uint32_t j) {
-    const float4 q = pos[j];
-    return make_float4(q.x, q.y, q.z, omega[j].w);
-  };
-  const bool staged = stage_tile(t, TL, lds4, gload);
+__global__ __launch_bounds__(kBlock) void k_confine(
+    uint32_t n, uint32_t nblocks, const float4 *__restrict__ pw,
+    const float4 *__restrict__ omega, const uint32_t *__restrict__ list,
+    const int *__restrict__ cnt, DevParams P, const float4 *__restrict__ vin,
+    float4 *__restrict__ vout, const uint32_t *__restrict__ id_s, float4 *__restrict__ x_can,
+    float4 *__restrict__ v_can) {
+  const uint32_t s = particle_index(nblocks);
   if (s >= n) return;
-  const float4 p = pos[s];
-  const int c = TL.cnt[s];
+  const float4 p = pw[s];
+  const uint32_t *L = list_ptr(list, s);
+  const int c = cnt[s];
   float ex = 0.0f, ey = 0.0f, ez = 0.0f;
-  visit(staged, TL, s, c, lds4, gload, [&](const float4 &q) {
+  // only x_j and |omega_j| of the neighbour are needed
+  for_neighbors<float4>(L, c, s, [&](uint32_t j) { return pw[j]; }, [&](const float4 &q) {
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     const float r2c = fmaxf(dx * dx + dy * dy + dz * dz, P.rmin2);
     const float ir = fast_rsq(r2c);
@@ -1172,37 +982,18 @@ __global__ void k_zero_f64(double *p) { *p = 0.0; }
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-constexpr size_t kLds4 = sizeof(float4) * kCap;  // one staged float4 per particle
-constexpr size_t kLds8 = sizeof(Pair4) * kCap;   // two
-
-#define LAUNCH_TIMED(ctx, kid, ...)           \
-  do {                                        \
-    hipEvent_t ev_ = nullptr;                 \
-    prof_begin(ctx, kid, &ev_);               \
-    __VA_ARGS__;                              \
-    prof_end(ctx, kid, ev_);                  \
+#define LAUNCH_TIMED(ctx, kid, grid, ...)                               \
+  do {                                                                  \
+    hipEvent_t ev_ = nullptr;                                           \
+    prof_begin(ctx, kid, &ev_);                                         \
+    __VA_ARGS__;                                                        \
+    prof_end(ctx, kid, ev_);                                            \
   } while (0)
-
-static TileLists tile_lists(const sph_ctx *c) {
-  return TileLists{c->list16, c->list, c->list_cnt, c->tile_meta, c->tile_int, c->ntiles};
-}
-
-int configure_kernels() {
-  static int done = 0;
-  if (done) return SPH_OK;
-  const void *big[] = {(const void *)k_surface, (const void *)k_xsph,
-                       (const void *)k_vorticity, (const void *)k_warm_lambda<true>,
-                       (const void *)k_warm_lambda<false>};
-  for (const void *f : big)
-    SPH_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLds8));
-  done = 1;
-  return SPH_OK;
-}
 
 int launch_gravity(sph_ctx *c) {
   const uint32_t n = c->n;
   if (!n) return SPH_OK;
-  LAUNCH_TIMED(c, K_GRAVITY,
+  LAUNCH_TIMED(c, K_GRAVITY, 0,
                k_gravity<<<blocks_for(n), kBlock, 0, c->stream>>>(n, c->v_can, c->dev));
   SPH_HIP(hipGetLastError());
   return SPH_OK;
@@ -1211,7 +1002,7 @@ int launch_gravity(sph_ctx *c) {
 int launch_predict(sph_ctx *c) {
   const uint32_t n = c->n;
   if (!n) return SPH_OK;
-  LAUNCH_TIMED(c, K_PREDICT,
+  LAUNCH_TIMED(c, K_PREDICT, 0,
                k_predict<<<blocks_for(n), kBlock, 0, c->stream>>>(n, c->x_can, c->v_can,
                                                                   c->xs_can, c->dev));
   SPH_HIP(hipGetLastError());
@@ -1225,12 +1016,12 @@ int launch_neighbor_search(sph_ctx *c, bool fused) {
   hipStream_t s = c->stream;
   if (n) {
     if (fused) {
-      LAUNCH_TIMED(c, K_PREDICT_HASH,
+      LAUNCH_TIMED(c, K_PREDICT_HASH, 0,
                    k_predict_hash<<<blocks_for(n), kBlock, 0, s>>>(
                        n, c->x_can, c->v_can, c->xs_can, c->keys, c->cell_count, c->dev,
                        c->stats));
     } else {
-      LAUNCH_TIMED(c, K_HASH,
+      LAUNCH_TIMED(c, K_HASH, 0,
                    k_hash<<<blocks_for(n), kBlock, 0, s>>>(n, c->xs_can, c->keys,
                                                            c->cell_count, c->dev, c->stats));
     }
@@ -1250,20 +1041,22 @@ int launch_neighbor_search(sph_ctx *c, bool fused) {
   // cell_start[c] = #keys < c (lower bound), cell_start[cells] = n; re-zeroes counts
   rc = exclusive_scan(c->cell_count, c->cell_start, c->cells, true, true, &c->scan, s, c);
   if (rc) return rc;
-  c->pos = c->pos_a;
-  c->ntiles = blocks_for(n, kTile);
   if (n) {
-    LAUNCH_TIMED(c, K_REORDER,
+    c->pos = c->pos_a;
+    LAUNCH_TIMED(c, K_REORDER, 0,
                  k_reorder<<<blocks_for(n), kBlock, 0, s>>>(n, c->id_s, c->xs_can, c->x_can,
                                                             c->pos, c->xold));
     SPH_HIP(hipGetLastError());
-    LAUNCH_TIMED(c, K_BUILD_LISTS,
-                 k_build_lists<<<c->ntiles, kTile, kLds4, s>>>(
-                     n, c->ntiles, c->pos, c->key_s, c->cell_start, c->dev, c->list16, c->list,
-                     c->list_cnt, c->tile_meta, c->tile_int, c->stats, c->pair_partial));
+    const uint32_t nb = blocks_for(n);
+    LAUNCH_TIMED(c, K_BUILD_LISTS, 0,
+                 k_build_lists<<<nb, kBlock, 0, s>>>(n, nb, c->pos, c->key_s, c->cell_start,
+                                                     c->dev, c->list, c->list_cnt, c->stats,
+                                                     c->pair_partial));
+    c->pair_blocks = nb;
     SPH_HIP(hipGetLastError());
+  } else {
+    c->pos = c->pos_a;
   }
-  c->pair_blocks = c->ntiles;
   c->vel = c->vel_a;
   c->sorted_valid = true;
   c->lists_valid = true;
@@ -1280,26 +1073,27 @@ static float stiffness_scale(const sph_ctx *c, int substep) {
 int launch_density(sph_ctx *c, int mode, int err_slot) {
   const uint32_t n = c->n;
   if (!n) return SPH_OK;
-  const TileLists TL = tile_lists(c);
+  const uint32_t nb = blocks_for(n);
   hipStream_t s = c->stream;
-  const uint32_t nb = c->ntiles;
   if (mode == 0)
-    LAUNCH_TIMED(c, K_DENSITY_LAMBDA,
-                 k_density_lambda<0><<<nb, kTile, kLds4, s>>>(n, TL, c->pos, c->dev, c->rho,
-                                                              c->stats, err_slot,
-                                                              c->P.density_tolerance > 0, 0));
+    LAUNCH_TIMED(c, K_DENSITY_LAMBDA, 0,
+                 k_density_lambda<0><<<nb, kBlock, 0, s>>>(n, nb, c->pos, c->list, c->list_cnt,
+                                                           c->dev, c->rho, c->stats, err_slot,
+                                                           c->P.density_tolerance > 0, 0));
   else if (mode == 1)
-    LAUNCH_TIMED(c, K_DENSITY,
-                 k_density_lambda<1><<<nb, kTile, kLds4, s>>>(n, TL, c->pos, c->dev, c->rho,
-                                                              c->stats, err_slot, 0, 0));
-  else if (mode == 3)  // density with rho also into w (final check / staged path)
-    LAUNCH_TIMED(c, K_DENSITY,
-                 k_density_lambda<1><<<nb, kTile, kLds4, s>>>(n, TL, c->pos, c->dev, c->rho,
-                                                              c->stats, err_slot, 1, 1));
+    LAUNCH_TIMED(c, K_DENSITY, 0,
+                 k_density_lambda<1><<<nb, kBlock, 0, s>>>(n, nb, c->pos, c->list, c->list_cnt,
+                                                           c->dev, c->rho, c->stats, err_slot, 0,
+                                                           0));
+  else if (mode == 3)  // final density of a capped tolerance loop: rho also into w
+    LAUNCH_TIMED(c, K_DENSITY, 0,
+                 k_density_lambda<1><<<nb, kBlock, 0, s>>>(n, nb, c->pos, c->list, c->list_cnt,
+                                                           c->dev, c->rho, c->stats, err_slot, 1,
+                                                           1));
   else
-    LAUNCH_TIMED(c, K_LAMBDA,
-                 k_density_lambda<2><<<nb, kTile, kLds4, s>>>(n, TL, c->pos, c->dev, c->rho,
-                                                              c->stats, -1, 0, 0));
+    LAUNCH_TIMED(c, K_LAMBDA, 0,
+                 k_density_lambda<2><<<nb, kBlock, 0, s>>>(n, nb, c->pos, c->list, c->list_cnt,
+                                                           c->dev, c->rho, c->stats, -1, 0, 0));
   SPH_HIP(hipGetLastError());
   return SPH_OK;
 }
@@ -1307,9 +1101,10 @@ int launch_density(sph_ctx *c, int mode, int err_slot) {
 int launch_dp(sph_ctx *c, int substep) {
   const uint32_t n = c->n;
   if (!n) return SPH_OK;
-  LAUNCH_TIMED(c, K_DP,
-               k_dp<<<c->ntiles, kTile, kLds4, c->stream>>>(
-                   n, tile_lists(c), c->pos, c->dev, stiffness_scale(c, substep), c->dpbuf));
+  const uint32_t nb = blocks_for(n);
+  LAUNCH_TIMED(c, K_DP, 0,
+               k_dp<<<nb, kBlock, 0, c->stream>>>(n, nb, c->pos, c->list, c->list_cnt, c->dev,
+                                                  stiffness_scale(c, substep), c->dpbuf));
   SPH_HIP(hipGetLastError());
   return SPH_OK;
 }
@@ -1317,7 +1112,7 @@ int launch_dp(sph_ctx *c, int substep) {
 int launch_position_stars(sph_ctx *c) {
   const uint32_t n = c->n;
   if (!n) return SPH_OK;
-  LAUNCH_TIMED(c, K_POS_STARS,
+  LAUNCH_TIMED(c, K_POS_STARS, 0,
                k_position_stars<<<blocks_for(n), kBlock, 0, c->stream>>>(
                    n, c->pos, c->dpbuf, c->rho, c->dev, c->sdf));
   SPH_HIP(hipGetLastError());
@@ -1331,12 +1126,14 @@ static float stiffness_unitless(const sph_ctx *c, int substep) {
 int launch_dp_update(sph_ctx *c, int substep, int err_slot, bool accumulate) {
   const uint32_t n = c->n;
   if (!n) return SPH_OK;
+  const uint32_t nb = blocks_for(n);
   float4 *out = (c->pos == c->pos_a) ? c->pos_b : c->pos_a;
-  LAUNCH_TIMED(c, K_DP_UPDATE,
-               k_dp_update<<<c->ntiles, kTile, kLds4, c->stream>>>(
-                   n, tile_lists(c), c->pos, out, c->rho, c->dev, stiffness_scale(c, substep),
-                   c->sdf, c->stats, err_slot, (float)c->P.density_tolerance,
-                   accumulate ? c->lam_acc : nullptr, stiffness_unitless(c, substep)));
+  LAUNCH_TIMED(c, K_DP_UPDATE, 0,
+               k_dp_update<<<nb, kBlock, 0, c->stream>>>(
+                   n, nb, c->pos, out, c->rho, c->list, c->list_cnt, c->dev,
+                   stiffness_scale(c, substep), c->sdf, c->stats, err_slot,
+                   (float)c->P.density_tolerance, accumulate ? c->lam_acc : nullptr,
+                   stiffness_unitless(c, substep)));
   SPH_HIP(hipGetLastError());
   c->pos = out;
   return SPH_OK;
@@ -1345,17 +1142,29 @@ int launch_dp_update(sph_ctx *c, int substep, int err_slot, bool accumulate) {
 int launch_warm_lambda(sph_ctx *c, bool log) {
   const uint32_t n = c->n;
   if (!n) return SPH_OK;
-  const TileLists TL = tile_lists(c);
+  const uint32_t nb = blocks_for(n);
   if (log)
-    LAUNCH_TIMED(c, K_WARM_LAMBDA,
-                 k_warm_lambda<true><<<c->ntiles, kTile, kLds8, c->stream>>>(
-                     n, TL, c->pos, c->xold, c->dev, c->rho, c->stats, c->id_s, c->lam_prev,
-                     c->sdf, nullptr, c->lam_acc, c->feat));
+    LAUNCH_TIMED(c, K_WARM_LAMBDA, 0,
+                 k_warm_lambda<true><<<nb, kBlock, 0, c->stream>>>(
+                     n, nb, c->pos, c->xold, c->list, c->list_cnt, c->dev, c->rho, c->stats,
+                     c->id_s, c->lam_prev, c->sdf, nullptr, c->lam_acc, c->feat, nullptr));
   else
-    LAUNCH_TIMED(c, K_WARM_LAMBDA,
-                 k_warm_lambda<false><<<c->ntiles, kTile, kLds8, c->stream>>>(
-                     n, TL, c->pos, c->xold, c->dev, c->rho, c->stats, c->id_s, c->lam_prev,
-                     c->sdf, c->warm_net, c->lam_acc, nullptr));
+    LAUNCH_TIMED(c, K_WARM_LAMBDA, 0,
+                 k_warm_lambda<false><<<nb, kBlock, 0, c->stream>>>(
+                     n, nb, c->pos, c->xold, c->list, c->list_cnt, c->dev, c->rho, c->stats,
+                     c->id_s, c->lam_prev, c->sdf, c->warm_net, c->lam_acc, nullptr,
+                     reinterpret_cast<float2 *>(c->dpbuf)));
+  SPH_HIP(hipGetLastError());
+  return SPH_OK;
+}
+
+int launch_warm_fallback(sph_ctx *c, const float4 *pre_warm) {
+  const uint32_t n = c->n;
+  if (!n) return SPH_OK;
+  LAUNCH_TIMED(c, K_AUX, 0,
+               k_warm_fallback<<<blocks_for(n), kBlock, 0, c->stream>>>(
+                   n, c->stats, pre_warm, c->pos, reinterpret_cast<const float2 *>(c->dpbuf),
+                   c->rho, c->lam_acc));
   SPH_HIP(hipGetLastError());
   return SPH_OK;
 }
@@ -1363,8 +1172,9 @@ int launch_warm_lambda(sph_ctx *c, bool log) {
 int launch_scatter_lambda(sph_ctx *c) {
   const uint32_t n = c->n;
   if (!n) return SPH_OK;
-  LAUNCH_TIMED(c, K_AUX,
-               k_scatter_lambda<<<blocks_for(n), kBlock, 0, c->stream>>>(n, c->id_s, c->lam_acc,
+  LAUNCH_TIMED(c, K_AUX, 0,
+               k_scatter_lambda<<<blocks_for(n), kBlock, 0, c->stream>>>(n, c->id_s,
+                                                                         c->lam_acc,
                                                                          c->lam_prev));
   SPH_HIP(hipGetLastError());
   return SPH_OK;
@@ -1378,11 +1188,12 @@ int launch_warm_samples(sph_ctx *c, float *x, float *t, uint32_t count, int ever
   return SPH_OK;
 }
 
+
 int launch_velocity(sph_ctx *c) {
   const uint32_t n = c->n;
   c->vel = c->vel_a;
   if (!n) return SPH_OK;
-  LAUNCH_TIMED(c, K_VELOCITY,
+  LAUNCH_TIMED(c, K_VELOCITY, 0,
                k_velocity<<<blocks_for(n), kBlock, 0, c->stream>>>(n, c->pos, c->xold, c->vel,
                                                                    c->dev));
   SPH_HIP(hipGetLastError());
@@ -1393,15 +1204,17 @@ int launch_color(sph_ctx *c, bool with_velocity) {
   const uint32_t n = c->n;
   if (with_velocity) c->vel = c->vel_a;
   if (!n) return SPH_OK;
-  const TileLists TL = tile_lists(c);
+  const uint32_t nb = blocks_for(n);
   if (with_velocity)
-    LAUNCH_TIMED(c, K_COLOR,
-                 k_color<true><<<c->ntiles, kTile, kLds4, c->stream>>>(n, TL, c->pos, c->xold,
-                                                                       c->dev, c->vel, c->aux));
+    LAUNCH_TIMED(c, K_COLOR, 0,
+                 k_color<true><<<nb, kBlock, 0, c->stream>>>(n, nb, c->pos, c->xold, c->list,
+                                                             c->list_cnt, c->dev, c->vel,
+                                                             c->aux));
   else
-    LAUNCH_TIMED(c, K_COLOR,
-                 k_color<false><<<c->ntiles, kTile, kLds4, c->stream>>>(
-                     n, TL, c->pos, c->xold, c->dev, c->vel, c->aux));
+    LAUNCH_TIMED(c, K_COLOR, 0,
+                 k_color<false><<<nb, kBlock, 0, c->stream>>>(n, nb, c->pos, c->xold, c->list,
+                                                              c->list_cnt, c->dev, c->vel,
+                                                              c->aux));
   SPH_HIP(hipGetLastError());
   return SPH_OK;
 }
@@ -1409,9 +1222,10 @@ int launch_color(sph_ctx *c, bool with_velocity) {
 int launch_surface(sph_ctx *c) {
   const uint32_t n = c->n;
   if (!n) return SPH_OK;
-  LAUNCH_TIMED(c, K_SURFACE,
-               k_surface<<<c->ntiles, kTile, kLds8, c->stream>>>(n, tile_lists(c), c->pos,
-                                                                 c->aux, c->dev, c->vel));
+  const uint32_t nb = blocks_for(n);
+  LAUNCH_TIMED(c, K_SURFACE, 0,
+               k_surface<<<nb, kBlock, 0, c->stream>>>(n, nb, c->pos, c->aux, c->list,
+                                                       c->list_cnt, c->dev, c->vel));
   SPH_HIP(hipGetLastError());
   return SPH_OK;
 }
@@ -1420,9 +1234,10 @@ int launch_xsph(sph_ctx *c) {
   const uint32_t n = c->n;
   float4 *out = (c->vel == c->vel_a) ? c->vel_b : c->vel_a;
   if (n) {
-    LAUNCH_TIMED(c, K_XSPH,
-                 k_xsph<<<c->ntiles, kTile, kLds8, c->stream>>>(n, tile_lists(c), c->pos,
-                                                                c->vel, c->dev, out));
+    const uint32_t nb = blocks_for(n);
+    LAUNCH_TIMED(c, K_XSPH, 0,
+                 k_xsph<<<nb, kBlock, 0, c->stream>>>(n, nb, c->pos, c->vel, c->list,
+                                                      c->list_cnt, c->dev, out));
     SPH_HIP(hipGetLastError());
   }
   c->vel = out;
@@ -1432,9 +1247,10 @@ int launch_xsph(sph_ctx *c) {
 int launch_vorticity(sph_ctx *c) {
   const uint32_t n = c->n;
   if (!n) return SPH_OK;
-  LAUNCH_TIMED(c, K_VORTICITY,
-               k_vorticity<<<c->ntiles, kTile, kLds8, c->stream>>>(n, tile_lists(c), c->pos,
-                                                                   c->vel, c->dev, c->aux));
+  const uint32_t nb = blocks_for(n);
+  LAUNCH_TIMED(c, K_VORTICITY, 0,
+               k_vorticity<<<nb, kBlock, 0, c->stream>>>(n, nb, c->pos, c->vel, c->list,
+                                                         c->list_cnt, c->dev, c->aux, c->pw));
   SPH_HIP(hipGetLastError());
   return SPH_OK;
 }
@@ -1443,25 +1259,29 @@ int launch_confine(sph_ctx *c, bool commit) {
   const uint32_t n = c->n;
   float4 *out = (c->vel == c->vel_a) ? c->vel_b : c->vel_a;
   if (n) {
-    const TileLists TL = tile_lists(c);
+    const uint32_t nb = blocks_for(n);
     if (commit)
-      LAUNCH_TIMED(c, K_CONFINE,
-                   k_confine<true><<<c->ntiles, kTile, kLds4, c->stream>>>(
-                       n, TL, c->pos, c->aux, c->dev, c->vel, out, c->id_s, c->x_can, c->v_can));
+      LAUNCH_TIMED(c, K_CONFINE, 0,
+                   k_confine<true><<<nb, kBlock, 0, c->stream>>>(
+                       n, nb, c->pw, c->aux, c->list, c->list_cnt, c->dev, c->vel, out,
+                       c->id_s, c->x_can, c->v_can));
     else
-      LAUNCH_TIMED(c, K_CONFINE,
-                   k_confine<false><<<c->ntiles, kTile, kLds4, c->stream>>>(
-                       n, TL, c->pos, c->aux, c->dev, c->vel, out, c->id_s, c->x_can, c->v_can));
+      LAUNCH_TIMED(c, K_CONFINE, 0,
+                   k_confine<false><<<nb, kBlock, 0, c->stream>>>(
+                       n, nb, c->pw, c->aux, c->list, c->list_cnt, c->dev, c->vel, out,
+                       c->id_s, c->x_can, c->v_can));
     SPH_HIP(hipGetLastError());
   }
   if (!commit) c->vel = out;
   return SPH_OK;
 }
 
+int configure_kernels() { return SPH_OK; }
+
 int launch_commit(sph_ctx *c) {
   const uint32_t n = c->n;
   if (!n) return SPH_OK;
-  LAUNCH_TIMED(c, K_COMMIT,
+  LAUNCH_TIMED(c, K_COMMIT, 0,
                k_commit<<<blocks_for(n), kBlock, 0, c->stream>>>(n, c->pos, c->vel, c->id_s,
                                                                  c->x_can, c->v_can));
   SPH_HIP(hipGetLastError());

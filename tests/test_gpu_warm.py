@@ -219,10 +219,12 @@ def test_trained_network_saves_iterations():
                                  vorticity_model=1, solve_iterations=iters,
                                  density_tolerance=tol)
 
-    def block(rng):
+    def block(rng, floor=False):
         sides = rng.uniform(0.25, 0.5, 3)
         counts = [int(a / s) for a in sides]
         origin = [rng.uniform(0.0, box[i] - counts[i] * s) for i in range(3)]
+        if floor:
+            origin[2] = 0.0
         return sa.Block(*counts, *origin)
 
     ctx = sa.Context(params(10, 1e-3), box, 25 ** 3)
@@ -249,16 +251,70 @@ def test_trained_network_saves_iterations():
     l0, _ = net.gradient(xb, tb, rows)
     W.train(net, xb, tb, rows, 1500, 8192, 2e-3, 1, log=lambda m: None)
     l1, _ = net.gradient(xb, tb, rows)
+    # held-out block collapsing on the floor: compressed from the first substep
     rng = np.random.default_rng(999)
-    ctx.upload(sa.scene_lattice([block(rng)], s, jitter=0.01, seed=99))
+    ctx.upload(sa.scene_lattice([block(rng, floor=True)], s, jitter=0.01, seed=99))
     ctx.set_params(params(10, 1e-3))
-    ctx.step(20)
+    ctx.step(30)
     pos, vel = ctx.download()
     cmp = W.compare(ctx, net, pos, vel, params(10, 1e-3), 10, log=lambda m: None)
+    pl, wm = cmp["plain"], cmp["warm"]
     record("warm_gain_small", rows=rows, loss0=l0, loss1=l1,
-           plain_iters=cmp["plain"]["iters_per_substep"],
-           warm_iters=cmp["warm"]["iters_per_substep"],
-           plain_err=cmp["plain"]["max_density_error"],
-           warm_err=cmp["warm"]["max_density_error"])
+           plain_iters=pl["iters_per_substep"], warm_iters=wm["iters_per_substep"],
+           plain_final=pl["mean_final_error"], warm_final=wm["mean_final_error"],
+           plain_max=pl["max_density_error"], warm_max=wm["max_density_error"],
+           warm_fallbacks=wm["fallbacks"])
     assert l1 < 0.5 * l0
-    assert cmp["warm"]["iters_per_substep"] < cmp["plain"]["iters_per_substep"], cmp
+    assert pl["mean_final_error"] > 1e-3  # the scene is compressed: not vacuous
+    # same acceptance test, fewer iterations or a smaller error at the cap
+    assert (wm["iters_per_substep"] < pl["iters_per_substep"] or
+            wm["mean_final_error"] < 0.9 * pl["mean_final_error"]), cmp
+
+
+def test_bad_prediction_falls_back_to_plain():
+    """A network that predicts a huge multiplier everywhere makes the first
+    step worse; the solver must revert it and finish like the plain solve."""
+    sc, pos, vel = _scene(iters=6)
+    n = len(pos)
+    plain = sc.context(n)
+    plain.upload(pos, vel)
+    plain.step(1)
+    r_plain = plain.residuals()
+    net = sa.Mlp(seed=1)
+    theta = net.weights()
+    theta[:] = 0.0
+    theta[-1] = 50.0  # b3: C_eff = 50 for every particle
+    net.set_weights(theta)
+    warm = sc.context(n)
+    warm.upload(pos, vel)
+    warm.set_warm_start(net)
+    warm.step(1)
+    st = warm.stats()
+    r_warm = warm.residuals()
+    record("warm_fallback", fallback=st.warm_fallback, plain_final=float(r_plain[-1]),
+           warm_final=float(r_warm[-1]))
+    assert st.warm_fallback == 1
+    assert len(r_plain) == len(r_warm) == 6
+    assert abs(r_warm[0] - r_plain[0]) <= 1e-6   # same state before any update
+    assert r_warm[1] == r_warm[0]            # the reverted step's slot holds the pre-step error
+    assert r_warm[-1] <= 2.0 * r_plain[-1] + 1e-6
+
+
+def test_residuals_report_every_density_pass():
+    sc, pos, vel = _scene(iters=5)
+    n = len(pos)
+    ctx = sc.context(n)
+    ctx.upload(pos, vel)
+    ctx.step(1)
+    r = ctx.residuals()
+    assert len(r) == 5 and np.all(r >= 0)
+    assert r[-1] == ctx.stats().max_density_error
+    p = sc.gpu_params()
+    p.density_tolerance = 1e-6
+    p.solve_iterations = 4
+    ctx.set_params(p)
+    ctx.step(1)
+    r = ctx.residuals()
+    st = ctx.stats()
+    assert len(r) == st.iterations + 1 == 5   # capped: 4 updates + the final pass
+    assert r[-1] == st.max_density_error
