@@ -81,7 +81,7 @@ static const char *kKernelNames[K_COUNT] = {
     "predict_hash", "hash", "gravity", "predict", "radix_upsweep", "radix_downsweep",
     "scan", "reorder", "build_lists", "density_lambda", "density", "lambda", "dp",
     "dp_update", "position_stars", "velocity", "color", "surface", "xsph", "vorticity",
-    "confine", "commit", "aux", "warm_lambda"};
+    "confine", "commit", "aux", "warm_lambda", "collide"};
 
 // Derived FP32 constants (SetParameters, simulation.c:147-177; the constants
 // of particles.c named in DevParams).  Computed in double, rounded once.

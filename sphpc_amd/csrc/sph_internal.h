@@ -92,7 +92,7 @@ enum KernelId {
   K_PREDICT_HASH = 0, K_HASH, K_GRAVITY, K_PREDICT, K_RADIX_UP, K_RADIX_DOWN, K_SCAN,
   K_REORDER, K_BUILD_LISTS, K_DENSITY_LAMBDA, K_DENSITY, K_LAMBDA, K_DP, K_DP_UPDATE,
   K_POS_STARS, K_VELOCITY, K_COLOR, K_SURFACE, K_XSPH, K_VORTICITY, K_CONFINE, K_COMMIT,
-  K_AUX, K_WARM_LAMBDA, K_COUNT
+  K_AUX, K_WARM_LAMBDA, K_COLLIDE, K_COUNT
 };
 
 struct ScanScratch {
@@ -156,6 +156,10 @@ struct sph_ctx {
   bool warm_on = false;
   bool log_on = false;
   long long feat_substep = -1;  // substep count after the last logged substep
+  // launch choices (sph_autotune)
+  struct {
+    bool split_collide = true;  // obstacle push as its own pass after k_dp_update
+  } tune;
   // whole-substep HIP graph (sph_set_graph)
   bool graph_on = false;
   bool graph_valid = false;

@@ -35,23 +35,29 @@ __device__ __forceinline__ const uint32_t *list_ptr(const uint32_t *list, uint32
   return list + (size_t)(s >> 6) * kListStride + (s & 63u);
 }
 
-// Visits the c neighbours of a particle in list order.  Each round first
-// issues kBatch list loads, then the kBatch dependent gathers, then the
-// arithmetic, so every wave keeps kBatch loads in flight instead of one
-// dependent chain per neighbour.  Padding slots load the particle itself
-// (`self`, always a valid index) and are skipped by the body.
+// Visits the c neighbours of a particle in list order.  Each round issues the
+// kBatch dependent gathers of this batch, then the kBatch list loads of the
+// next batch, then the arithmetic: the list loads of batch k+1 are in flight
+// while batch k is gathered and used, so a batch costs one memory round trip
+// instead of two.  Padding slots load the particle itself (`self`, always a
+// valid index) and are skipped by the body.
 constexpr int kBatch = 8;
 
 template <typename T, typename Load, typename Body>
 __device__ __forceinline__ void for_neighbors(const uint32_t *L, int c, uint32_t self, Load load,
                                               Body body) {
-  for (int k0 = 0; k0 < c; k0 += kBatch) {
-    uint32_t j[kBatch];
+  uint32_t jn[kBatch];
 #pragma unroll
-    for (int b = 0; b < kBatch; ++b) j[b] = (k0 + b < c) ? L[(size_t)(k0 + b) * kWave] : self;
+  for (int b = 0; b < kBatch; ++b) jn[b] = (b < c) ? L[(size_t)b * kWave] : self;
+  for (int k0 = 0; k0 < c; k0 += kBatch) {
     T q[kBatch];
 #pragma unroll
-    for (int b = 0; b < kBatch; ++b) q[b] = load(j[b]);
+    for (int b = 0; b < kBatch; ++b) q[b] = load(jn[b]);
+#pragma unroll
+    for (int b = 0; b < kBatch; ++b) {
+      const int k = k0 + kBatch + b;
+      jn[b] = (k < c) ? L[(size_t)k * kWave] : self;
+    }
 #pragma unroll
     for (int b = 0; b < kBatch; ++b)
       if (k0 + b < c) body(q[b]);
@@ -503,9 +509,13 @@ __global__ __launch_bounds__(kBlock) void k_position_stars(uint32_t n, float4 *_
 }
 
 // Fused UpdateDPs + UpdatePositionStars: reads pin (x*, lambda), writes pout
-// (x*', rho).  In tolerance mode, once the density pass of this iteration is
+// (x*', rho).  COLLIDE false leaves the obstacle to k_collide_sorted: the
+// inlined trilinear gradient doubles the kernel's registers (109 vs 58 VGPRs)
+// and halves its occupancy, for a push that only particles inside the SDF box
+// need.  In tolerance mode, once the density pass of this iteration is
 // within tolerance, the particles are copied unchanged instead.  With lam_acc,
 // adds stiff * lambda_i to the particle's accumulated multiplier.
+template <bool COLLIDE>
 __global__ __launch_bounds__(kBlock) void k_dp_update(
     uint32_t n, uint32_t nblocks, const float4 *__restrict__ pin, float4 *__restrict__ pout,
     const float *__restrict__ rho, const uint32_t *__restrict__ list,
@@ -528,7 +538,7 @@ __global__ __launch_bounds__(kBlock) void k_dp_update(
   const float3 a = dp_sum(p, pin, list_ptr(list, s), cnt[s], s, P);
   float x = p.x + scale * a.x, y = p.y + scale * a.y, z = p.z + scale * a.z;
   apply_boundary(x, y, z, P);
-  collide(x, y, z, o);
+  if (COLLIDE) collide(x, y, z, o);
   pout[s] = make_float4(x, y, z, rho[s]);
   if (lam_acc) lam_acc[s] += stiff * p.w;
 }
@@ -955,6 +965,22 @@ __global__ __launch_bounds__(kBlock) void k_collide_soa(uint32_t n, float *__res
   z[i] = c;
 }
 
+// CalculateParticleCollisions over the sorted positions (after a k_dp_update
+// without the fused push).  Only particles inside the obstacle's box are
+// written.
+__global__ __launch_bounds__(kBlock) void k_collide_sorted(uint32_t n, float4 *__restrict__ pos,
+                                                           SdfDev o) {
+  const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= n) return;
+  const float4 p = pos[s];
+  if (!(p.x > o.wmin[0] && p.x < o.wmax[0] && p.y > o.wmin[1] && p.y < o.wmax[1] &&
+        p.z > o.wmin[2] && p.z < o.wmax[2]))
+    return;
+  float x = p.x, y = p.y, z = p.z;
+  collide(x, y, z, o);
+  if (x != p.x || y != p.y || z != p.z) pos[s] = make_float4(x, y, z, p.w);
+}
+
 // FP64 kinetic energy: per-block partial sums, then one atomic per block.
 __global__ __launch_bounds__(kBlock) void k_kinetic_energy(uint32_t n,
                                                            const float4 *__restrict__ v,
@@ -1128,13 +1154,26 @@ int launch_dp_update(sph_ctx *c, int substep, int err_slot, bool accumulate) {
   if (!n) return SPH_OK;
   const uint32_t nb = blocks_for(n);
   float4 *out = (c->pos == c->pos_a) ? c->pos_b : c->pos_a;
-  LAUNCH_TIMED(c, K_DP_UPDATE, 0,
-               k_dp_update<<<nb, kBlock, 0, c->stream>>>(
-                   n, nb, c->pos, out, c->rho, c->list, c->list_cnt, c->dev,
-                   stiffness_scale(c, substep), c->sdf, c->stats, err_slot,
-                   (float)c->P.density_tolerance, accumulate ? c->lam_acc : nullptr,
-                   stiffness_unitless(c, substep)));
+  float *acc = accumulate ? c->lam_acc : nullptr;
+  const float scale = stiffness_scale(c, substep), stiff = stiffness_unitless(c, substep);
+  const float tol = (float)c->P.density_tolerance;
+  const bool split = c->sdf.data && c->tune.split_collide;
+  if (split)
+    LAUNCH_TIMED(c, K_DP_UPDATE, 0,
+                 k_dp_update<false><<<nb, kBlock, 0, c->stream>>>(
+                     n, nb, c->pos, out, c->rho, c->list, c->list_cnt, c->dev, scale, c->sdf,
+                     c->stats, err_slot, tol, acc, stiff));
+  else
+    LAUNCH_TIMED(c, K_DP_UPDATE, 0,
+                 k_dp_update<true><<<nb, kBlock, 0, c->stream>>>(
+                     n, nb, c->pos, out, c->rho, c->list, c->list_cnt, c->dev, scale, c->sdf,
+                     c->stats, err_slot, tol, acc, stiff));
   SPH_HIP(hipGetLastError());
+  if (split) {
+    LAUNCH_TIMED(c, K_COLLIDE, 0,
+                 k_collide_sorted<<<blocks_for(n), kBlock, 0, c->stream>>>(n, out, c->sdf));
+    SPH_HIP(hipGetLastError());
+  }
   c->pos = out;
   return SPH_OK;
 }
