@@ -110,6 +110,24 @@ void sph_default_params(sph_params *p, double smoothing_radius);
 int sph_create(const sph_params *params, const sph_aabb *boundary, int max_particles,
                sph_ctx **out);
 int sph_destroy(sph_ctx *ctx);  /* FinalizeParticles/FinalizeNeighbors */
+
+/* Batched independent scenes (no counterpart in the reference, which runs one
+ * scene per job): every stage runs once over all scenes' particles, each scene
+ * with its own box [0, size) in its own coordinates, its own hash grid and
+ * obstacle, and the shared physical parameters.  sph_upload_state then takes
+ * all scenes' particles concatenated in scene order, and sph_download_state
+ * returns them the same way. */
+typedef struct sph_scene {
+  int particles;
+  double size_x, size_y, size_z;  /* box [0, size) */
+  sph_obstacle_desc obstacle;     /* .sdf NULL: no obstacle */
+} sph_scene;
+int sph_create_batch(const sph_params *params, int nscenes, const sph_scene *scenes,
+                     sph_ctx **out);
+int sph_batch_scene_count(sph_ctx *ctx, int *nscenes);
+/* Max density error max_i max(rho_i - rho0, 0) / rho0 of each scene at the
+ * last density pass. */
+int sph_batch_scene_errors(sph_ctx *ctx, float *max_err);
 int sph_set_params(sph_ctx *ctx, const sph_params *params);
 int sph_get_params(sph_ctx *ctx, sph_params *params);
 int sph_grid_dims(sph_ctx *ctx, int *nx, int *ny, int *nz);

@@ -53,6 +53,11 @@ class ObstacleDesc(C.Structure):
                 ("spacing", C.c_double), ("world_bounds", AABB), ("sdf", C.c_void_p)]
 
 
+class Scene(C.Structure):
+    _fields_ = [("particles", C.c_int), ("size_x", C.c_double), ("size_y", C.c_double),
+                ("size_z", C.c_double), ("obstacle", ObstacleDesc)]
+
+
 class Stats(C.Structure):
     _fields_ = [("particles", C.c_int), ("iterations", C.c_int), ("converged", C.c_int),
                 ("max_density_error", C.c_float), ("neighbor_overflow", C.c_ulonglong),
@@ -138,6 +143,9 @@ def lib():
     L.sph_mlp_load.argtypes = [vp, C.c_char_p]
     L.sph_set_warm_start.argtypes = [vp, vp]
     L.sph_set_graph.argtypes = [vp, ip]
+    L.sph_create_batch.argtypes = [C.POINTER(Params), ip, C.POINTER(Scene), C.POINTER(vp)]
+    L.sph_batch_scene_count.argtypes = [vp, C.POINTER(ip)]
+    L.sph_batch_scene_errors.argtypes = [vp, fp]
     L.sph_get_residuals.argtypes = [vp, fp, ip, C.POINTER(ip)]
     L.sph_mlp_set_output_clip.argtypes = [vp, C.c_float]
     L.sph_mlp_get_output_clip.argtypes = [vp]
@@ -350,6 +358,42 @@ class Context:
         ms, n = C.c_double(), C.c_longlong()
         check(lib().sph_profile_read(self._h, int(kernel), C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+
+class BatchContext(Context):
+    """Batched independent scenes in one context (sph_create_batch).
+
+    scenes: list of dicts with keys particles, size (3 floats) and optionally
+    sdf = dict(sdf=<flat float32 grid>, dims=(nx, ny, nz), spacing, wmin, wmax).
+    Positions are uploaded and downloaded concatenated in scene order, each
+    scene in its own coordinates (box [0, size))."""
+
+    def __init__(self, params, scenes):
+        self.params = params
+        arr = (Scene * len(scenes))()
+        self._keep = []
+        for k, sc in enumerate(scenes):
+            arr[k].particles = int(sc["particles"])
+            arr[k].size_x, arr[k].size_y, arr[k].size_z = [float(v) for v in sc["size"]]
+            o = sc.get("sdf")
+            if o is not None:
+                g = _f32(o["sdf"]).ravel()
+                self._keep.append(g)
+                wmin, wmax = o["wmin"], o["wmax"]
+                arr[k].obstacle = ObstacleDesc(
+                    int(o["dims"][0]), int(o["dims"][1]), int(o["dims"][2]), float(o["spacing"]),
+                    AABB(wmin[0], wmax[0], wmin[1], wmax[1], wmin[2], wmax[2]), g.ctypes.data)
+        self._h = C.c_void_p()
+        check(lib().sph_create_batch(C.byref(params), len(scenes), arr, C.byref(self._h)))
+        counts = [int(sc["particles"]) for sc in scenes]
+        self.capacity = sum(counts)
+        self.nscenes = len(scenes)
+        self.first = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+
+    def scene_errors(self):
+        out = np.zeros(self.nscenes, np.float32)
+        check(lib().sph_batch_scene_errors(self._h, _fptr(out)))
+        return out
 
 
 MLP_FEATURES = 12

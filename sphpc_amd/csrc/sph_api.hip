@@ -166,7 +166,8 @@ static void free_all(sph_ctx *c) {
                   c->hist, c->cell_count, c->cell_start, c->pos_a, c->pos_b, c->xold,
                   c->vel_a, c->vel_b, c->aux, c->dpbuf, c->rho, c->list, c->list_cnt,
                   c->stats, c->pair_partial, c->red, c->sdf_buf, c->scan.tile_sums,
-                  c->pw, c->lam_acc, c->lam_prev, c->feat,
+                  c->pw, c->lam_acc, c->lam_prev, c->feat, c->scenes_dev, c->scene_of,
+                  c->batch_sdf, c->scene_err,
                   c->warm_net};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
@@ -235,42 +236,10 @@ extern "C" void sph_default_params(sph_params *p, double h) {
   p->vorticity_model = 0;
 }
 
-extern "C" int sph_create(const sph_params *params, const sph_aabb *boundary, int max_particles,
-                          sph_ctx **out) {
-  if (!out) return arg_error("out is NULL");
-  *out = nullptr;
-  int rc = check_params(params);
-  if (rc) return rc;
-  if (!boundary) return arg_error("boundary is NULL");
-  if (boundary->min_x != 0.0 || boundary->min_y != 0.0 || boundary->min_z != 0.0)
-    return arg_error("boundary must start at the origin (the reference hashes absolute "
-                     "coordinates, neighbors.c:86-95)");
-  if (!(boundary->max_x > 0.0 && boundary->max_y > 0.0 && boundary->max_z > 0.0))
-    return arg_error("boundary extent must be positive");
-  if (max_particles < 0) return arg_error("max_particles must be >= 0");
-
-  sph_ctx *c = new sph_ctx();
-  c->P = *params;
-  c->box = *boundary;
-  const double h = params->smoothing_radius;
-  // AllocInitNeighbors, neighbors.c:23-28: ceil(L/h) + 1 cells per axis
-  const double nxd = ceil((boundary->max_x - boundary->min_x) / h) + 1;
-  const double nyd = ceil((boundary->max_y - boundary->min_y) / h) + 1;
-  const double nzd = ceil((boundary->max_z - boundary->min_z) / h) + 1;
-  if (nxd * nyd * nzd >= 4294967295.0) {
-    delete c;
-    return arg_error("hash grid exceeds 2^32 cells");
-  }
-  c->dev.nx = (uint32_t)nxd;
-  c->dev.ny = (uint32_t)nyd;
-  c->dev.nz = (uint32_t)nzd;
-  c->dev.nxy = c->dev.nx * c->dev.ny;
-  c->cells = c->dev.nxy * c->dev.nz;
-  c->key_bits = 1;
-  while (c->key_bits < 32 && ((uint64_t)1 << c->key_bits) < (uint64_t)c->cells) c->key_bits++;
-  derive(c);
-  c->cap = max_particles;
-
+// Device buffers, stream and kernel set-up of a context whose grid (cells,
+// key_bits) and capacity are known.  Frees the context on failure.
+static int ctx_allocate(sph_ctx *c, int max_particles, sph_ctx **out) {
+  int rc = SPH_OK;
   auto fail = [&](int code) {
     free_all(c);
     delete c;
@@ -284,6 +253,7 @@ extern "C" int sph_create(const sph_params *params, const sph_aabb *boundary, in
   if (e != hipSuccess) return fail(hip_fail(e, "hipStreamCreate"));
 
   const size_t cap = (size_t)std::max(max_particles, 1);
+  c->cap = max_particles;
   const size_t f4 = sizeof(float4) * cap, u4 = sizeof(uint32_t) * cap;
   const size_t groups = (cap + kWave - 1) / kWave;
   struct {
@@ -321,11 +291,167 @@ extern "C" int sph_create(const sph_params *params, const sph_aabb *boundary, in
   return SPH_OK;
 }
 
+extern "C" int sph_create(const sph_params *params, const sph_aabb *boundary, int max_particles,
+                          sph_ctx **out) {
+  if (!out) return arg_error("out is NULL");
+  *out = nullptr;
+  int rc = check_params(params);
+  if (rc) return rc;
+  if (!boundary) return arg_error("boundary is NULL");
+  if (boundary->min_x != 0.0 || boundary->min_y != 0.0 || boundary->min_z != 0.0)
+    return arg_error("boundary must start at the origin (the reference hashes absolute "
+                     "coordinates, neighbors.c:86-95)");
+  if (!(boundary->max_x > 0.0 && boundary->max_y > 0.0 && boundary->max_z > 0.0))
+    return arg_error("boundary extent must be positive");
+  if (max_particles < 0) return arg_error("max_particles must be >= 0");
+
+  sph_ctx *c = new sph_ctx();
+  c->P = *params;
+  c->box = *boundary;
+  const double h = params->smoothing_radius;
+  // AllocInitNeighbors, neighbors.c:23-28: ceil(L/h) + 1 cells per axis
+  const double nxd = ceil((boundary->max_x - boundary->min_x) / h) + 1;
+  const double nyd = ceil((boundary->max_y - boundary->min_y) / h) + 1;
+  const double nzd = ceil((boundary->max_z - boundary->min_z) / h) + 1;
+  if (nxd * nyd * nzd >= 4294967295.0) {
+    delete c;
+    return arg_error("hash grid exceeds 2^32 cells");
+  }
+  c->dev.nx = (uint32_t)nxd;
+  c->dev.ny = (uint32_t)nyd;
+  c->dev.nz = (uint32_t)nzd;
+  c->dev.nxy = c->dev.nx * c->dev.ny;
+  c->cells = c->dev.nxy * c->dev.nz;
+  c->key_bits = 1;
+  while (c->key_bits < 32 && ((uint64_t)1 << c->key_bits) < (uint64_t)c->cells) c->key_bits++;
+  derive(c);
+  return ctx_allocate(c, max_particles, out);
+}
+
 extern "C" int sph_destroy(sph_ctx *c) {
   if (!c) return SPH_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_all(c);
   delete c;
+  return SPH_OK;
+}
+
+static int sdf_fields(const sph_obstacle_desc *o, SdfDev *d);
+
+// Batched independent scenes in one context: one particle array, one sort and
+// one launch per stage for all scenes, each with its own box, grid (a
+// contiguous range of cell keys) and obstacle; scene k's particles are
+// indices [first_k, first_k + particles_k) in upload order and in the sorted
+// working order alike.
+extern "C" int sph_create_batch(const sph_params *params, int nscenes, const sph_scene *scenes,
+                                sph_ctx **out) {
+  if (!out) return arg_error("out is NULL");
+  *out = nullptr;
+  int rc = check_params(params);
+  if (rc) return rc;
+  if (nscenes < 1 || nscenes > 65535 || !scenes)
+    return arg_error("need 1..65535 scenes");
+  const double h = params->smoothing_radius;
+  std::vector<SceneDev> hs((size_t)nscenes);
+  std::vector<uint32_t> first((size_t)nscenes + 1, 0u);
+  uint64_t cells = 0, total = 0;
+  size_t sdf_words = 0;
+  double ext[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < nscenes; ++k) {
+    const sph_scene &sc = scenes[k];
+    const double L[3] = {sc.size_x, sc.size_y, sc.size_z};
+    if (sc.particles < 0) return arg_error("scene particle count must be >= 0");
+    if (!(L[0] > 0.0 && L[1] > 0.0 && L[2] > 0.0)) return arg_error("scene size must be positive");
+    SceneDev &d = hs[k];
+    memset(&d, 0, sizeof(d));
+    // AllocInitNeighbors, neighbors.c:23-28: ceil(L/h) + 1 cells per axis
+    d.nx = (uint32_t)(ceil(L[0] / h) + 1);
+    d.ny = (uint32_t)(ceil(L[1] / h) + 1);
+    d.nz = (uint32_t)(ceil(L[2] / h) + 1);
+    d.nxy = d.nx * d.ny;
+    d.cell_base = (uint32_t)cells;
+    cells += (uint64_t)d.nxy * d.nz;
+    for (int a = 0; a < 3; ++a) {
+      d.box_hi[a] = (float)L[a];
+      d.box_hi_clamp[a] = (float)(L[a] - h * 0.0001);
+      ext[a] = std::max(ext[a], L[a]);
+    }
+    if (sc.obstacle.sdf) {
+      if ((rc = sdf_fields(&sc.obstacle, &d.sdf))) return rc;
+      sdf_words += (size_t)sc.obstacle.dim_x * sc.obstacle.dim_y * sc.obstacle.dim_z;
+    }
+    total += (uint64_t)sc.particles;
+    first[k + 1] = (uint32_t)std::min<uint64_t>(total, 0xffffffffu);
+    if (cells >= 0xffffffffull) return arg_error("hash grids exceed 2^32 cells");
+  }
+  if (total > 0x7fffffffull) return arg_error("too many particles");
+  sph_ctx *c = new sph_ctx();
+  c->P = *params;
+  c->box = sph_aabb{0.0, ext[0], 0.0, ext[1], 0.0, ext[2]};
+  c->cells = (uint32_t)cells;
+  c->key_bits = 1;
+  while (c->key_bits < 32 && ((uint64_t)1 << c->key_bits) < cells) c->key_bits++;
+  c->dev.nx = c->dev.ny = c->dev.nz = c->dev.nxy = 1;  // per scene instead
+  derive(c);
+  rc = ctx_allocate(c, (int)total, out);
+  if (rc) return rc;
+  // scene table, per-particle scene index, obstacle grids
+  const size_t np = std::max<size_t>((size_t)total, 1);
+  std::vector<uint16_t> owner(np, 0);
+  for (int k = 0; k < nscenes; ++k)
+    for (uint32_t i = first[k]; i < first[k + 1]; ++i) owner[i] = (uint16_t)k;
+  hipError_t e = hipMalloc((void **)&c->scenes_dev, sizeof(SceneDev) * nscenes);
+  if (e == hipSuccess) e = hipMalloc((void **)&c->scene_of, sizeof(uint16_t) * np);
+  if (e == hipSuccess) e = hipMalloc((void **)&c->scene_err, sizeof(uint32_t) * nscenes);
+  if (e == hipSuccess && sdf_words) e = hipMalloc((void **)&c->batch_sdf, sizeof(float) * sdf_words);
+  size_t off = 0;
+  for (int k = 0; k < nscenes && e == hipSuccess; ++k) {
+    const sph_obstacle_desc &o = scenes[k].obstacle;
+    if (!o.sdf) continue;
+    const size_t cnt = (size_t)o.dim_x * o.dim_y * o.dim_z;
+    hs[k].sdf.data = c->batch_sdf + off;
+    e = hipMemcpyAsync(c->batch_sdf + off, o.sdf, sizeof(float) * cnt, hipMemcpyHostToDevice,
+                       c->stream);
+    off += cnt;
+  }
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(c->scenes_dev, hs.data(), sizeof(SceneDev) * nscenes,
+                       hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(c->scene_of, owner.data(), sizeof(uint16_t) * np, hipMemcpyHostToDevice,
+                       c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    const int code = hip_fail(e, "batch set-up");
+    sph_ctx *done = *out;
+    *out = nullptr;
+    free_all(done);
+    delete done;
+    return code;
+  }
+  c->nscenes = nscenes;
+  c->scene_first = first;
+  return SPH_OK;
+}
+
+extern "C" int sph_batch_scene_count(sph_ctx *c, int *nscenes) {
+  if (!c || !nscenes) return arg_error("NULL argument");
+  *nscenes = c->nscenes;
+  return SPH_OK;
+}
+
+extern "C" int sph_batch_scene_errors(sph_ctx *c, float *max_err) {
+  if (!c || !max_err) return arg_error("NULL argument");
+  if (!c->nscenes) return state_error("not a batched context");
+  int rc = require_lists(c);
+  if (rc) return rc;
+  rc = launch_scene_errors(c);
+  if (rc) return rc;
+  std::vector<uint32_t> w((size_t)c->nscenes);
+  SPH_HIP(hipMemcpyAsync(w.data(), c->scene_err, sizeof(uint32_t) * w.size(),
+                         hipMemcpyDeviceToHost, c->stream));
+  SPH_HIP(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < c->nscenes; ++k) memcpy(&max_err[k], &w[k], sizeof(float));
   return SPH_OK;
 }
 
@@ -355,17 +481,14 @@ extern "C" int sph_grid_dims(sph_ctx *c, int *nx, int *ny, int *nz) {
   return SPH_OK;
 }
 
-static int make_sdf(const sph_obstacle_desc *o, SdfDev *d, float **buf, hipStream_t s) {
+// Fields of an obstacle grid (AllocInitObstacle_CUDA, obstacle_cuda.cu:
+// 133-183) except the device pointer.
+static int sdf_fields(const sph_obstacle_desc *o, SdfDev *d) {
   if (o->dim_x < 1 || o->dim_y < 1 || o->dim_z < 1 || !(o->spacing > 0.0))
     return arg_error("obstacle dims and spacing must be positive");
   const sph_aabb &b = o->world_bounds;
   if (!(b.max_x > b.min_x && b.max_y > b.min_y && b.max_z > b.min_z))
     return arg_error("obstacle world bounds must be non-empty");
-  const size_t count = (size_t)o->dim_x * (size_t)o->dim_y * (size_t)o->dim_z;
-  SPH_HIP(hipMalloc(buf, sizeof(float) * count));
-  SPH_HIP(hipMemcpyAsync(*buf, o->sdf, sizeof(float) * count, hipMemcpyHostToDevice, s));
-  SPH_HIP(hipStreamSynchronize(s));
-  d->data = *buf;
   d->dim[0] = o->dim_x;
   d->dim[1] = o->dim_y;
   d->dim[2] = o->dim_z;
@@ -381,8 +504,20 @@ static int make_sdf(const sph_obstacle_desc *o, SdfDev *d, float **buf, hipStrea
   return SPH_OK;
 }
 
+static int make_sdf(const sph_obstacle_desc *o, SdfDev *d, float **buf, hipStream_t s) {
+  int rc = sdf_fields(o, d);
+  if (rc) return rc;
+  const size_t count = (size_t)o->dim_x * (size_t)o->dim_y * (size_t)o->dim_z;
+  SPH_HIP(hipMalloc(buf, sizeof(float) * count));
+  SPH_HIP(hipMemcpyAsync(*buf, o->sdf, sizeof(float) * count, hipMemcpyHostToDevice, s));
+  SPH_HIP(hipStreamSynchronize(s));
+  d->data = *buf;
+  return SPH_OK;
+}
+
 extern "C" int sph_set_obstacle(sph_ctx *c, const sph_obstacle_desc *o) {
   if (!c) return arg_error("context is NULL");
+  if (c->nscenes) return state_error("a batched context takes its obstacles from sph_create_batch");
   graph_reset(c);
   if (c->sdf_buf) {
     SPH_HIP(hipStreamSynchronize(c->stream));
@@ -410,6 +545,8 @@ extern "C" int sph_upload_state(sph_ctx *c, int n, const float *x, const float *
     return SPH_ERR_CAPACITY;
   }
   if (n > 0 && (!x || !y || !z || !vx || !vy || !vz)) return arg_error("NULL state array");
+  if (c->nscenes && n != c->cap)
+    return arg_error("a batched context takes all scenes' particles at once, in scene order");
   if (n != c->n) graph_reset(c);
   std::vector<float4> hx((size_t)n), hv((size_t)n);
   for (int i = 0; i < n; ++i) {

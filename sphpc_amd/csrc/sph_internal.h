@@ -54,6 +54,21 @@ struct SdfDev {
   float m2w;          // world length / model length (obstacle_cuda.cu:101-105)
 };
 
+// Batched scenes: each scene has its own box (local coordinates from the
+// origin), hash grid (a contiguous range of global cell keys from cell_base)
+// and obstacle.  A scene's particles are contiguous in the id order and, as
+// its keys are contiguous, at the same indices in the sorted order.
+struct SceneDev {
+  uint32_t cell_base, nx, ny, nz, nxy;
+  float box_hi[3], box_hi_clamp[3];
+  SdfDev sdf;
+};
+
+struct BatchDev {
+  const SceneDev *scenes;     // nullptr: one scene described by DevParams
+  const uint16_t *scene_of;   // scene of particle index i (both orders)
+};
+
 enum StatSlot {
   STAT_DROPPED = 0,      // 64-bit counter (two words)
   STAT_CLAMPS = 2,       // 64-bit counter (two words)
@@ -156,6 +171,14 @@ struct sph_ctx {
   bool warm_on = false;
   bool log_on = false;
   long long feat_substep = -1;  // substep count after the last logged substep
+  // batched scenes (sph_create_batch); nscenes == 0: single scene
+  int nscenes = 0;
+  sph::SceneDev *scenes_dev = nullptr;
+  uint16_t *scene_of = nullptr;
+  float *batch_sdf = nullptr;
+  uint32_t *scene_err = nullptr;
+  std::vector<uint32_t> scene_first;  // nscenes + 1 particle offsets
+  sph::BatchDev batch() const { return sph::BatchDev{scenes_dev, scene_of}; }
   // launch choices (sph_autotune)
   struct {
     bool split_collide = true;  // obstacle push as its own pass after k_dp_update
@@ -231,6 +254,7 @@ int launch_warm_lambda(sph_ctx *c, bool log);
 int launch_scatter_lambda(sph_ctx *c);
 int launch_warm_samples(sph_ctx *c, float *x, float *t, uint32_t count, int every);
 int launch_warm_fallback(sph_ctx *c, const float4 *pre_warm);
+int launch_scene_errors(sph_ctx *c);
 int mlp_pack(sph_mlp *m, MlpPacked *dst, hipStream_t s);  // sph_mlp.hip
 int configure_kernels();  // LDS limits of the tiled kernels, once per process
 

@@ -78,6 +78,51 @@ __device__ __forceinline__ float clamp_axis(float x, float lo, float hi, float h
   return x;
 }
 
+// Box and hash grid of the scene a particle belongs to (the context's own for
+// a single scene; the scene table's in a batch).  Boxes start at the origin.
+struct Geo {
+  uint32_t base, nx, ny, nz, nxy;
+  float hi[3], hi_clamp[3];
+};
+
+template <bool BATCH>
+__device__ __forceinline__ Geo geo_of(uint32_t i, const DevParams &P, const BatchDev &B) {
+  Geo g;
+  if (!BATCH) {
+    g.base = 0u;
+    g.nx = P.nx; g.ny = P.ny; g.nz = P.nz; g.nxy = P.nxy;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      g.hi[a] = P.box_hi[a];
+      g.hi_clamp[a] = P.box_hi_clamp[a];
+    }
+  } else {
+    const SceneDev &d = B.scenes[B.scene_of[i]];
+    g.base = d.cell_base;
+    g.nx = d.nx; g.ny = d.ny; g.nz = d.nz; g.nxy = d.nxy;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      g.hi[a] = d.box_hi[a];
+      g.hi_clamp[a] = d.box_hi_clamp[a];
+    }
+  }
+  return g;
+}
+
+// Obstacle of particle i's scene.
+template <bool BATCH>
+__device__ __forceinline__ const SdfDev &sdf_of(uint32_t i, const SdfDev &o, const BatchDev &B) {
+  if (!BATCH) return o;
+  return B.scenes[B.scene_of[i]].sdf;
+}
+
+__device__ __forceinline__ void apply_boundary(float &x, float &y, float &z, const Geo &g,
+                                               const DevParams &P) {
+  x = clamp_axis(x, P.box_lo[0], g.hi[0], g.hi_clamp[0]);
+  y = clamp_axis(y, P.box_lo[1], g.hi[1], g.hi_clamp[1]);
+  z = clamp_axis(z, P.box_lo[2], g.hi[2], g.hi_clamp[2]);
+}
+
 __device__ __forceinline__ void apply_boundary(float &x, float &y, float &z,
                                                const DevParams &P) {
   x = clamp_axis(x, P.box_lo[0], P.box_hi[0], P.box_hi_clamp[0]);
@@ -107,13 +152,13 @@ __device__ __forceinline__ uint32_t cell_coord(float x, float h, float inv_h, ui
   return (uint32_t)k;
 }
 
-// HashVal (neighbors.c:79-98).
+// HashVal (neighbors.c:79-98), offset by the scene's first cell key.
 __device__ __forceinline__ uint32_t hash_key(float x, float y, float z, const DevParams &P,
-                                             uint32_t *clamped) {
-  const uint32_t gx = cell_coord(x, P.h, P.inv_h, P.nx, clamped);
-  const uint32_t gy = cell_coord(y, P.h, P.inv_h, P.ny, clamped);
-  const uint32_t gz = cell_coord(z, P.h, P.inv_h, P.nz, clamped);
-  return P.nxy * gz + (gy * P.nx + gx);
+                                             const Geo &g, uint32_t *clamped) {
+  const uint32_t gx = cell_coord(x, P.h, P.inv_h, g.nx, clamped);
+  const uint32_t gy = cell_coord(y, P.h, P.inv_h, g.ny, clamped);
+  const uint32_t gz = cell_coord(z, P.h, P.inv_h, g.nz, clamped);
+  return g.base + (g.nxy * gz + (gy * g.nx + gx));
 }
 
 __device__ __forceinline__ void add_counter64(uint32_t *stats, int slot, uint32_t v) {
@@ -149,28 +194,31 @@ __global__ __launch_bounds__(kBlock) void k_gravity(uint32_t n, float4 *__restri
 }
 
 // PredictPositions (particles.c:748-784).
+template <bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_predict(uint32_t n, const float4 *__restrict__ x,
                                                     const float4 *__restrict__ v,
-                                                    float4 *__restrict__ xs, DevParams P) {
+                                                    float4 *__restrict__ xs, DevParams P,
+                                                    BatchDev B) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const float4 a = x[i], u = v[i];
   float px = a.x + u.x * P.dt, py = a.y + u.y * P.dt, pz = a.z + u.z * P.dt;
-  apply_boundary(px, py, pz, P);
+  apply_boundary(px, py, pz, geo_of<BATCH>(i, P, B), P);
   xs[i] = make_float4(px, py, pz, 0.0f);
 }
 
 // HashParticles (neighbors.c:101-125) plus the cell histogram used for the
 // cell tables.  count must be zero on entry (the scan re-zeroes it).
+template <bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_hash(uint32_t n, const float4 *__restrict__ xs,
                                                  uint32_t *__restrict__ keys,
                                                  uint32_t *__restrict__ count, DevParams P,
-                                                 uint32_t *__restrict__ stats) {
+                                                 uint32_t *__restrict__ stats, BatchDev B) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const float4 a = xs[i];
   uint32_t clamped = 0;
-  const uint32_t key = hash_key(a.x, a.y, a.z, P, &clamped);
+  const uint32_t key = hash_key(a.x, a.y, a.z, P, geo_of<BATCH>(i, P, B), &clamped);
   keys[i] = key;
   atomicAdd(&count[key], 1u);
   add_counter64(stats, STAT_CLAMPS, clamped);
@@ -179,6 +227,7 @@ __global__ __launch_bounds__(kBlock) void k_hash(uint32_t n, const float4 *__res
 // Fused ApplyGravity + PredictPositions + HashParticles for the fused
 // substep: v after gravity is only read by the prediction (UpdateVelocities
 // overwrites it), so it is not stored.
+template <bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_predict_hash(uint32_t n,
                                                          const float4 *__restrict__ x,
                                                          const float4 *__restrict__ v,
@@ -186,16 +235,18 @@ __global__ __launch_bounds__(kBlock) void k_predict_hash(uint32_t n,
                                                          uint32_t *__restrict__ keys,
                                                          uint32_t *__restrict__ count,
                                                          DevParams P,
-                                                         uint32_t *__restrict__ stats) {
+                                                         uint32_t *__restrict__ stats,
+                                                         BatchDev B) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const float4 a = x[i], u = v[i];
   const float vz = u.z + P.g_dt;
   float px = a.x + u.x * P.dt, py = a.y + u.y * P.dt, pz = a.z + vz * P.dt;
-  apply_boundary(px, py, pz, P);
+  const Geo g = geo_of<BATCH>(i, P, B);
+  apply_boundary(px, py, pz, g, P);
   xs[i] = make_float4(px, py, pz, 0.0f);
   uint32_t clamped = 0;
-  const uint32_t key = hash_key(px, py, pz, P, &clamped);
+  const uint32_t key = hash_key(px, py, pz, P, g, &clamped);
   keys[i] = key;
   atomicAdd(&count[key], 1u);
   add_counter64(stats, STAT_CLAMPS, clamped);
@@ -243,11 +294,12 @@ __device__ __noinline__ bool inside_exact(float px, float py, float pz, float qx
 // two grid rows of particles) writes its hits directly.
 constexpr uint32_t kLayerSpan = 1u << 14;
 
+template <bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_build_lists(
     uint32_t n, uint32_t nblocks, const float4 *__restrict__ pos,
     const uint32_t *__restrict__ key_s, const uint32_t *__restrict__ cstart, DevParams P,
     uint32_t *__restrict__ list, int *__restrict__ cnt, uint32_t *__restrict__ stats,
-    uint32_t *__restrict__ pair_partial) {
+    uint32_t *__restrict__ pair_partial, BatchDev B) {
   __shared__ uint16_t sbuf[kMaxNb][kBlock];
   const uint32_t s = particle_index(nblocks);
   int c = 0;
@@ -257,32 +309,33 @@ __global__ __launch_bounds__(kBlock) void k_build_lists(
   uint32_t *out = list + (size_t)(s >> 6) * kListStride + (s & 63u);
   if (s < n) {
     const float4 p = pos[s];
-    const uint32_t key = key_s[s];
-    const uint32_t cz = key / P.nxy;
-    const uint32_t rem = key - cz * P.nxy;
-    const uint32_t cy = rem / P.nx;
-    const uint32_t cx = rem - cy * P.nx;
+    const Geo G = geo_of<BATCH>(s, P, B);
+    const uint32_t key = key_s[s] - G.base;
+    const uint32_t cz = key / G.nxy;
+    const uint32_t rem = key - cz * G.nxy;
+    const uint32_t cy = rem / G.nx;
+    const uint32_t cx = rem - cy * G.nx;
     const uint32_t x0 = cx > 0 ? cx - 1 : 0;
-    const uint32_t x1 = cx + 1 < P.nx ? cx + 1 : P.nx - 1;
+    const uint32_t x1 = cx + 1 < G.nx ? cx + 1 : G.nx - 1;
     const uint32_t y0 = cy > 0 ? cy - 1 : 0;
-    const uint32_t y1 = cy + 1 < P.ny ? cy + 1 : P.ny - 1;
+    const uint32_t y1 = cy + 1 < G.ny ? cy + 1 : G.ny - 1;
 #pragma unroll
     for (int l = 0; l < 3; ++l) {
       const int z = (int)cz + l - 1;
-      if (z < 0 || z >= (int)P.nz) continue;
-      const uint32_t zrow = (uint32_t)z * P.nxy;
-      base[l] = cstart[zrow + y0 * P.nx + x0];
-      if (cstart[zrow + y1 * P.nx + x1 + 1] - base[l] > kLayerSpan) direct = true;
+      if (z < 0 || z >= (int)G.nz) continue;
+      const uint32_t zrow = G.base + (uint32_t)z * G.nxy;
+      base[l] = cstart[zrow + y0 * G.nx + x0];
+      if (cstart[zrow + y1 * G.nx + x1 + 1] - base[l] > kLayerSpan) direct = true;
     }
     for (int dz = -1; dz <= 1; ++dz) {
       const int z = (int)cz + dz;
-      if (z < 0 || z >= (int)P.nz) continue;
+      if (z < 0 || z >= (int)G.nz) continue;
       const uint32_t lb = dz < 0 ? base[0] : (dz == 0 ? base[1] : base[2]);
       const uint32_t tag = (uint32_t)(dz + 1) << 14;
       for (int dy = -1; dy <= 1; ++dy) {
         const int y = (int)cy + dy;
-        if (y < 0 || y >= (int)P.ny) continue;
-        const uint32_t row = (uint32_t)z * P.nxy + (uint32_t)y * P.nx;
+        if (y < 0 || y >= (int)G.ny) continue;
+        const uint32_t row = G.base + (uint32_t)z * G.nxy + (uint32_t)y * G.nx;
         const uint32_t j0 = cstart[row + x0];
         const uint32_t j1 = cstart[row + x1 + 1];
         // candidates in batches of kBatch loads; hits are appended in j order
@@ -495,16 +548,17 @@ __global__ __launch_bounds__(kBlock) void k_dp(uint32_t n, uint32_t nblocks,
 
 // Staged UpdatePositionStars (particles.c:529-564): x* += dp, clamp, collide.
 // Also stores rho in w for the velocity passes.
+template <bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_position_stars(uint32_t n, float4 *__restrict__ pos,
                                                            const float4 *__restrict__ dp,
                                                            const float *__restrict__ rho,
-                                                           DevParams P, SdfDev o) {
+                                                           DevParams P, SdfDev o, BatchDev B) {
   const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
   if (s >= n) return;
   const float4 p = pos[s], d = dp[s];
   float x = p.x + d.x, y = p.y + d.y, z = p.z + d.z;
-  apply_boundary(x, y, z, P);
-  collide(x, y, z, o);
+  apply_boundary(x, y, z, geo_of<BATCH>(s, P, B), P);
+  collide(x, y, z, sdf_of<BATCH>(s, o, B));
   pos[s] = make_float4(x, y, z, rho[s]);
 }
 
@@ -515,12 +569,12 @@ __global__ __launch_bounds__(kBlock) void k_position_stars(uint32_t n, float4 *_
 // need.  In tolerance mode, once the density pass of this iteration is
 // within tolerance, the particles are copied unchanged instead.  With lam_acc,
 // adds stiff * lambda_i to the particle's accumulated multiplier.
-template <bool COLLIDE>
+template <bool COLLIDE, bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_dp_update(
     uint32_t n, uint32_t nblocks, const float4 *__restrict__ pin, float4 *__restrict__ pout,
     const float *__restrict__ rho, const uint32_t *__restrict__ list,
     const int *__restrict__ cnt, DevParams P, float scale, SdfDev o, uint32_t *stats,
-    int err_slot, float tol, float *__restrict__ lam_acc, float stiff) {
+    int err_slot, float tol, float *__restrict__ lam_acc, float stiff, BatchDev B) {
   bool done = false;
   if (tol > 0.0f) {
     done = converged(stats) ||
@@ -537,8 +591,8 @@ __global__ __launch_bounds__(kBlock) void k_dp_update(
   if (s == 0) stats[STAT_ITERS] += 1u;
   const float3 a = dp_sum(p, pin, list_ptr(list, s), cnt[s], s, P);
   float x = p.x + scale * a.x, y = p.y + scale * a.y, z = p.z + scale * a.z;
-  apply_boundary(x, y, z, P);
-  if (COLLIDE) collide(x, y, z, o);
+  apply_boundary(x, y, z, geo_of<BATCH>(s, P, B), P);
+  if (COLLIDE) collide(x, y, z, sdf_of<BATCH>(s, o, B));
   pout[s] = make_float4(x, y, z, rho[s]);
   if (lam_acc) lam_acc[s] += stiff * p.w;
 }
@@ -579,13 +633,14 @@ __device__ __forceinline__ void sdf_features(float x, float y, float z, const Sd
 // and writes Lambda_pred = -max(C_eff, 0) / (|grad C|^2 + eps) as lambda.
 // Either way the particle's accumulated multiplier restarts at zero.  Every
 // lane of a wave reaches the network (no early return).
-template <bool LOG>
+template <bool LOG, bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_warm_lambda(
     uint32_t n, uint32_t nblocks, float4 *pos, const float4 *__restrict__ xold,
     const uint32_t *__restrict__ list, const int *__restrict__ cnt, DevParams P,
     float *__restrict__ rho_io, uint32_t *stats, const uint32_t *__restrict__ id_s,
     const float *__restrict__ lam_prev, SdfDev o, const MlpPacked *__restrict__ net,
-    float *__restrict__ lam_acc, float *__restrict__ feat, float2 *__restrict__ plain) {
+    float *__restrict__ lam_acc, float *__restrict__ feat, float2 *__restrict__ plain,
+    BatchDev B) {
   const uint32_t s = particle_index(nblocks);
   const bool active = s < n;
   float f[kMlpPad];
@@ -633,7 +688,7 @@ __global__ __launch_bounds__(kBlock) void k_warm_lambda(
     f[2] = -lam_prev[id_s[s]] * denom;
     f[3] = gc2 / denom;
     f[4] = P.div_k * dv;
-    sdf_features(p.x, p.y, p.z, o, P, f + 5);
+    sdf_features(p.x, p.y, p.z, sdf_of<BATCH>(s, o, B), P, f + 5);
     const float mk = P.mWn / (rho * P.h);
     f[9] = mk * mx;
     f[10] = mk * my;
@@ -968,10 +1023,13 @@ __global__ __launch_bounds__(kBlock) void k_collide_soa(uint32_t n, float *__res
 // CalculateParticleCollisions over the sorted positions (after a k_dp_update
 // without the fused push).  Only particles inside the obstacle's box are
 // written.
+template <bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_collide_sorted(uint32_t n, float4 *__restrict__ pos,
-                                                           SdfDev o) {
+                                                           SdfDev o0, BatchDev B) {
   const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
   if (s >= n) return;
+  const SdfDev &o = sdf_of<BATCH>(s, o0, B);
+  if (!o.data) return;
   const float4 p = pos[s];
   if (!(p.x > o.wmin[0] && p.x < o.wmax[0] && p.y > o.wmin[1] && p.y < o.wmax[1] &&
         p.z > o.wmin[2] && p.z < o.wmax[2]))
@@ -979,6 +1037,36 @@ __global__ __launch_bounds__(kBlock) void k_collide_sorted(uint32_t n, float4 *_
   float x = p.x, y = p.y, z = p.z;
   collide(x, y, z, o);
   if (x != p.x || y != p.y || z != p.z) pos[s] = make_float4(x, y, z, p.w);
+}
+
+// Max density error per scene of a batch, from the last density pass (rho in
+// sorted order; a scene's particles are contiguous there).  A wave whose
+// lanes all belong to one scene issues one atomic; waves that straddle a
+// scene boundary issue one per lane.
+__global__ __launch_bounds__(kBlock) void k_scene_errors(uint32_t n, const float *__restrict__ rho,
+                                                         const uint16_t *__restrict__ scene_of,
+                                                         DevParams P, uint32_t *err_bits) {
+  const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+  const bool active = s < n;
+  const uint32_t k = active ? scene_of[s] : 0xffffffffu;
+  const float e = active ? fmaxf((rho[s] - P.rest_density) * P.inv_rest_density, 0.0f) : 0.0f;
+  const uint32_t k0 = (uint32_t)__shfl(k, 0, 64);
+  if (__all(!active || k == k0)) {
+    float m = e;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0 && active && m > 0.0f) {
+      const uint32_t bits = __float_as_uint(m);
+      if (bits > *(volatile uint32_t *)&err_bits[k0]) atomicMax(&err_bits[k0], bits);
+    }
+  } else if (active && e > 0.0f) {
+    atomicMax(&err_bits[k], __float_as_uint(e));
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_words(uint32_t n, uint32_t *p, uint32_t v) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) p[i] = v;
 }
 
 // FP64 kinetic energy: per-block partial sums, then one atomic per block.
@@ -1028,9 +1116,15 @@ int launch_gravity(sph_ctx *c) {
 int launch_predict(sph_ctx *c) {
   const uint32_t n = c->n;
   if (!n) return SPH_OK;
-  LAUNCH_TIMED(c, K_PREDICT, 0,
-               k_predict<<<blocks_for(n), kBlock, 0, c->stream>>>(n, c->x_can, c->v_can,
-                                                                  c->xs_can, c->dev));
+  const BatchDev B = c->batch();
+  if (c->nscenes)
+    LAUNCH_TIMED(c, K_PREDICT, 0,
+                 k_predict<true><<<blocks_for(n), kBlock, 0, c->stream>>>(
+                     n, c->x_can, c->v_can, c->xs_can, c->dev, B));
+  else
+    LAUNCH_TIMED(c, K_PREDICT, 0,
+                 k_predict<false><<<blocks_for(n), kBlock, 0, c->stream>>>(
+                     n, c->x_can, c->v_can, c->xs_can, c->dev, B));
   SPH_HIP(hipGetLastError());
   return SPH_OK;
 }
@@ -1040,16 +1134,26 @@ int launch_predict(sph_ctx *c) {
 int launch_neighbor_search(sph_ctx *c, bool fused) {
   const uint32_t n = c->n;
   hipStream_t s = c->stream;
+  const BatchDev B = c->batch();
   if (n) {
-    if (fused) {
+    if (fused && c->nscenes) {
       LAUNCH_TIMED(c, K_PREDICT_HASH, 0,
-                   k_predict_hash<<<blocks_for(n), kBlock, 0, s>>>(
+                   k_predict_hash<true><<<blocks_for(n), kBlock, 0, s>>>(
                        n, c->x_can, c->v_can, c->xs_can, c->keys, c->cell_count, c->dev,
-                       c->stats));
+                       c->stats, B));
+    } else if (fused) {
+      LAUNCH_TIMED(c, K_PREDICT_HASH, 0,
+                   k_predict_hash<false><<<blocks_for(n), kBlock, 0, s>>>(
+                       n, c->x_can, c->v_can, c->xs_can, c->keys, c->cell_count, c->dev,
+                       c->stats, B));
+    } else if (c->nscenes) {
+      LAUNCH_TIMED(c, K_HASH, 0,
+                   k_hash<true><<<blocks_for(n), kBlock, 0, s>>>(
+                       n, c->xs_can, c->keys, c->cell_count, c->dev, c->stats, B));
     } else {
       LAUNCH_TIMED(c, K_HASH, 0,
-                   k_hash<<<blocks_for(n), kBlock, 0, s>>>(n, c->xs_can, c->keys,
-                                                           c->cell_count, c->dev, c->stats));
+                   k_hash<false><<<blocks_for(n), kBlock, 0, s>>>(
+                       n, c->xs_can, c->keys, c->cell_count, c->dev, c->stats, B));
     }
     SPH_HIP(hipGetLastError());
   }
@@ -1074,10 +1178,16 @@ int launch_neighbor_search(sph_ctx *c, bool fused) {
                                                             c->pos, c->xold));
     SPH_HIP(hipGetLastError());
     const uint32_t nb = blocks_for(n);
-    LAUNCH_TIMED(c, K_BUILD_LISTS, 0,
-                 k_build_lists<<<nb, kBlock, 0, s>>>(n, nb, c->pos, c->key_s, c->cell_start,
-                                                     c->dev, c->list, c->list_cnt, c->stats,
-                                                     c->pair_partial));
+    if (c->nscenes)
+      LAUNCH_TIMED(c, K_BUILD_LISTS, 0,
+                   k_build_lists<true><<<nb, kBlock, 0, s>>>(
+                       n, nb, c->pos, c->key_s, c->cell_start, c->dev, c->list, c->list_cnt,
+                       c->stats, c->pair_partial, B));
+    else
+      LAUNCH_TIMED(c, K_BUILD_LISTS, 0,
+                   k_build_lists<false><<<nb, kBlock, 0, s>>>(
+                       n, nb, c->pos, c->key_s, c->cell_start, c->dev, c->list, c->list_cnt,
+                       c->stats, c->pair_partial, B));
     c->pair_blocks = nb;
     SPH_HIP(hipGetLastError());
   } else {
@@ -1138,9 +1248,14 @@ int launch_dp(sph_ctx *c, int substep) {
 int launch_position_stars(sph_ctx *c) {
   const uint32_t n = c->n;
   if (!n) return SPH_OK;
-  LAUNCH_TIMED(c, K_POS_STARS, 0,
-               k_position_stars<<<blocks_for(n), kBlock, 0, c->stream>>>(
-                   n, c->pos, c->dpbuf, c->rho, c->dev, c->sdf));
+  if (c->nscenes)
+    LAUNCH_TIMED(c, K_POS_STARS, 0,
+                 k_position_stars<true><<<blocks_for(n), kBlock, 0, c->stream>>>(
+                     n, c->pos, c->dpbuf, c->rho, c->dev, c->sdf, c->batch()));
+  else
+    LAUNCH_TIMED(c, K_POS_STARS, 0,
+                 k_position_stars<false><<<blocks_for(n), kBlock, 0, c->stream>>>(
+                     n, c->pos, c->dpbuf, c->rho, c->dev, c->sdf, c->batch()));
   SPH_HIP(hipGetLastError());
   return SPH_OK;
 }
@@ -1157,21 +1272,30 @@ int launch_dp_update(sph_ctx *c, int substep, int err_slot, bool accumulate) {
   float *acc = accumulate ? c->lam_acc : nullptr;
   const float scale = stiffness_scale(c, substep), stiff = stiffness_unitless(c, substep);
   const float tol = (float)c->P.density_tolerance;
-  const bool split = c->sdf.data && c->tune.split_collide;
-  if (split)
-    LAUNCH_TIMED(c, K_DP_UPDATE, 0,
-                 k_dp_update<false><<<nb, kBlock, 0, c->stream>>>(
-                     n, nb, c->pos, out, c->rho, c->list, c->list_cnt, c->dev, scale, c->sdf,
-                     c->stats, err_slot, tol, acc, stiff));
-  else
-    LAUNCH_TIMED(c, K_DP_UPDATE, 0,
-                 k_dp_update<true><<<nb, kBlock, 0, c->stream>>>(
-                     n, nb, c->pos, out, c->rho, c->list, c->list_cnt, c->dev, scale, c->sdf,
-                     c->stats, err_slot, tol, acc, stiff));
+  const BatchDev B = c->batch();
+  const bool obstacle = c->nscenes ? c->batch_sdf != nullptr : c->sdf.data != nullptr;
+  const bool split = obstacle && c->tune.split_collide;
+#define DP_UPDATE(COL, BAT)                                                              \
+  LAUNCH_TIMED(c, K_DP_UPDATE, 0,                                                        \
+               (k_dp_update<COL, BAT><<<nb, kBlock, 0, c->stream>>>(                     \
+                   n, nb, c->pos, out, c->rho, c->list, c->list_cnt, c->dev, scale, c->sdf, \
+                   c->stats, err_slot, tol, acc, stiff, B)))
+  if (c->nscenes) {
+    if (split) DP_UPDATE(false, true); else DP_UPDATE(true, true);
+  } else {
+    if (split) DP_UPDATE(false, false); else DP_UPDATE(true, false);
+  }
+#undef DP_UPDATE
   SPH_HIP(hipGetLastError());
   if (split) {
-    LAUNCH_TIMED(c, K_COLLIDE, 0,
-                 k_collide_sorted<<<blocks_for(n), kBlock, 0, c->stream>>>(n, out, c->sdf));
+    if (c->nscenes)
+      LAUNCH_TIMED(c, K_COLLIDE, 0,
+                   k_collide_sorted<true><<<blocks_for(n), kBlock, 0, c->stream>>>(
+                       n, out, c->sdf, B));
+    else
+      LAUNCH_TIMED(c, K_COLLIDE, 0,
+                   k_collide_sorted<false><<<blocks_for(n), kBlock, 0, c->stream>>>(
+                       n, out, c->sdf, B));
     SPH_HIP(hipGetLastError());
   }
   c->pos = out;
@@ -1182,17 +1306,20 @@ int launch_warm_lambda(sph_ctx *c, bool log) {
   const uint32_t n = c->n;
   if (!n) return SPH_OK;
   const uint32_t nb = blocks_for(n);
-  if (log)
-    LAUNCH_TIMED(c, K_WARM_LAMBDA, 0,
-                 k_warm_lambda<true><<<nb, kBlock, 0, c->stream>>>(
-                     n, nb, c->pos, c->xold, c->list, c->list_cnt, c->dev, c->rho, c->stats,
-                     c->id_s, c->lam_prev, c->sdf, nullptr, c->lam_acc, c->feat, nullptr));
-  else
-    LAUNCH_TIMED(c, K_WARM_LAMBDA, 0,
-                 k_warm_lambda<false><<<nb, kBlock, 0, c->stream>>>(
-                     n, nb, c->pos, c->xold, c->list, c->list_cnt, c->dev, c->rho, c->stats,
-                     c->id_s, c->lam_prev, c->sdf, c->warm_net, c->lam_acc, nullptr,
-                     reinterpret_cast<float2 *>(c->dpbuf)));
+  const BatchDev B = c->batch();
+  float2 *plain = reinterpret_cast<float2 *>(c->dpbuf);
+#define WARM(LOG, BAT)                                                                      \
+  LAUNCH_TIMED(c, K_WARM_LAMBDA, 0,                                                         \
+               (k_warm_lambda<LOG, BAT><<<nb, kBlock, 0, c->stream>>>(                       \
+                   n, nb, c->pos, c->xold, c->list, c->list_cnt, c->dev, c->rho, c->stats,   \
+                   c->id_s, c->lam_prev, c->sdf, LOG ? nullptr : c->warm_net, c->lam_acc,   \
+                   LOG ? c->feat : nullptr, LOG ? nullptr : plain, B)))
+  if (c->nscenes) {
+    if (log) WARM(true, true); else WARM(false, true);
+  } else {
+    if (log) WARM(true, false); else WARM(false, false);
+  }
+#undef WARM
   SPH_HIP(hipGetLastError());
   return SPH_OK;
 }
@@ -1205,6 +1332,19 @@ int launch_warm_fallback(sph_ctx *c, const float4 *pre_warm) {
                    n, c->stats, pre_warm, c->pos, reinterpret_cast<const float2 *>(c->dpbuf),
                    c->rho, c->lam_acc));
   SPH_HIP(hipGetLastError());
+  return SPH_OK;
+}
+
+int launch_scene_errors(sph_ctx *c) {
+  const uint32_t ns = (uint32_t)c->nscenes;
+  if (!ns) return SPH_OK;
+  k_fill_words<<<blocks_for(ns), kBlock, 0, c->stream>>>(ns, c->scene_err, 0u);
+  SPH_HIP(hipGetLastError());
+  if (c->n) {
+    k_scene_errors<<<blocks_for(c->n), kBlock, 0, c->stream>>>(c->n, c->rho, c->scene_of, c->dev,
+                                                                c->scene_err);
+    SPH_HIP(hipGetLastError());
+  }
   return SPH_OK;
 }
 
