@@ -45,21 +45,8 @@ KERNEL_BYTES = {
 
 
 def scene_cfg2(scale=1.0):
-    import sphpc_amd as sa
-    s = float(np.float32(0.008))
-    nx, ny, nz = (max(1, int(round(256 * scale))), max(1, int(round(128 * scale))),
-                  max(1, int(round(128 * scale))))
-    y0 = (2.0 - ny * s) / 2.0
-    blocks = [sa.Block(nx, ny, nz, 0.0, y0, 0.0),
-              sa.Block(nx, ny, nz, 6.0 - nx * s, y0, 0.0)]
-    pos = sa.scene_lattice(blocks, s, jitter=0.01, seed=2025)
-    dims = (256, 256, 256)
-    wmin, wmax = (2.2, 0.2, 0.0), (3.8, 1.8, 1.6)
-    shapes = [sa.torus((3.0, 0.75, 0.12), 0.4, 0.12, axis=2),
-              sa.box_shape((3.0, 1.45, 0.25), (0.25, 0.2, 0.25))]
-    sdf = sa.sdf_sample_shapes(shapes, dims, wmin, wmax)
-    return dict(s=s, box=(6.0, 2.0, 2.0), pos=pos, sdf=sdf, dims=dims, wmin=wmin, wmax=wmax,
-                dx=(wmax[0] - wmin[0]) / dims[0])
+    from sphpc_amd import workloads
+    return workloads.double_dam_break(scale)
 
 
 def gpu_params(sc, iters=4, tol=0.0):
@@ -113,6 +100,91 @@ def cpu_baseline(sc, substeps, threads):
                        "reported baseline only")
 
 
+def timed_region(ctx, steps):
+    """Times `steps` fused substeps; every kernel launch is bracketed by HIP
+    events on the library's own stream, whose sums give per-kernel averages."""
+    import sphpc_amd as sa
+    ctx.synchronize()
+    ctx.profile(True)
+    ctx.profile_reset()
+    t0 = time.perf_counter()
+    ctx.step(steps)
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    times = {}
+    for k, name in enumerate(sa.kernel_names()):
+        ms, cnt = ctx.profile_read(k)
+        if cnt:
+            times[name] = (ms, cnt)
+    ctx.profile(False)
+    return elapsed, times
+
+
+def roofline(times, n, nbar, steps, tag):
+    """Roofline object of the dominant gather kernel plus per-kernel ms per substep."""
+    gathers = {k: v for k, v in times.items() if k in KERNEL_BYTES}
+    dom = max(gathers, key=lambda k: gathers[k][0])
+    ms, cnt = gathers[dom]
+    avg_s = ms / cnt / 1e3
+    L = 4.0 * (nbar + 1.0)
+    bytes_launch = KERNEL_BYTES[dom](L) * n
+    achieved = bytes_launch / avg_s / 1e9
+    roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(dom, tag),
+            "algorithmic_bytes_per_launch": bytes_launch, "avg_launch_ms": avg_s * 1e3,
+            "launches": cnt}
+    per = {k: v[0] / steps for k, v in sorted(times.items(), key=lambda kv: -kv[1][0])}
+    per["_sum"] = sum(v[0] for v in times.values()) / steps
+    return roof, per
+
+
+def bench_batch(args):
+    """Batched scenes: `--scenes` independent scenes in one context, one launch
+    per stage for all of them (sphpc_amd/workloads.py)."""
+    import sphpc_amd as sa
+    from sphpc_amd import workloads
+    s, scenes, pos = workloads.batched_scenes(args.scenes, seed=4)
+    n = len(pos)
+    p = sa.default_params(2.0 * s, time_step=1.0 / 300.0, g=9.81, k=0.5, c=0.01,
+                          rest_density=1000.0, rest_mass=1000.0 * s ** 3, lambda_epsilon=600.0,
+                          vorticity_epsilon=0.01, vorticity_model=1, solve_iterations=4)
+    ctx = sa.BatchContext(p, scenes)
+    ctx.upload(pos)
+    ctx.set_graph(not args.no_graph)
+    ctx.step(args.warmup)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.step(args.steps)
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    errs = ctx.scene_errors()
+    st = ctx.stats()
+    nbar = st.neighbor_pairs / max(n, 1)
+    prof_steps = max(2, args.steps // 5)
+    _, times = timed_region(ctx, prof_steps)
+    roof, per = roofline(times, n, nbar, prof_steps, f"batch{args.scenes}")
+    counts = [sc["particles"] for sc in scenes]
+    out = {
+        "metric": f"particle-updates/s, {args.scenes} batched scenes",
+        "value": n * args.steps / elapsed, "unit": "particle-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.scenes} independent seeded scenes (2k-128k particles, "
+                               "box aspect 1-4, fill 0.1-0.5, sphere SDF 64^3), s=0.01 m, "
+                               "4 Jacobi iterations, surface tension + XSPH + vorticity",
+                   "particles": n, "scenes": args.scenes, "min_scene": int(min(counts)),
+                   "max_scene": int(max(counts)), "graph": not args.no_graph,
+                   "mean_neighbors": nbar,
+                   "scene_max_density_error": {"mean": float(np.mean(errs)),
+                                               "median": float(np.median(errs)),
+                                               "max": float(np.max(errs))}},
+        "roofline": roof, "kernels_ms_per_substep_eager": per,
+    }
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +199,12 @@ def main():
                     help="only the timed substeps (for profiler runs)")
     ap.add_argument("--net", default=os.path.join(ROOT, "sphpc_amd", "data", "warm_net_s0.008.bin"),
                     help="trained warm-start network (sphpc_amd/warm_train.py)")
+    ap.add_argument("--workload", choices=["dam8m", "batch"], default="dam8m",
+                    help="dam8m: the 8M double dam break (the headline line); "
+                         "batch: independent batched scenes")
+    ap.add_argument("--scenes", type=int, default=512)
+    ap.add_argument("--no-graph", action="store_true",
+                    help="batch workload: eager launches instead of a captured graph")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run two 1 GiB device copies (counter calibration)")
     args = ap.parse_args()
@@ -146,6 +224,10 @@ def main():
         src.free()
         dst.free()
 
+    if args.workload == "batch":
+        bench_batch(args)
+        return
+
     sc = scene_cfg2(args.scale)
     n = len(sc["pos"])
     ctx = sa.Context(gpu_params(sc), sc["box"], n)
@@ -153,22 +235,7 @@ def main():
     ctx.set_obstacle(sc["sdf"], sc["dims"], sc["dx"], sc["wmin"], sc["wmax"])
 
     ctx.step(args.warmup)
-    ctx.synchronize()
-    # Timed region.  Every kernel launch is bracketed by HIP events recorded on
-    # the library's own stream; their sums give the per-kernel averages.
-    ctx.profile(True)
-    ctx.profile_reset()
-    t0 = time.perf_counter()
-    ctx.step(args.steps)
-    ctx.synchronize()
-    elapsed = time.perf_counter() - t0
-    names = sa.kernel_names()
-    times = {}
-    for k, name in enumerate(names):
-        ms, cnt = ctx.profile_read(k)
-        if cnt:
-            times[name] = (ms, cnt)
-    ctx.profile(False)
+    elapsed, times = timed_region(ctx, args.steps)
     value = n * args.steps / elapsed
     st = ctx.stats()
     nbar = st.neighbor_pairs / max(n, 1)
@@ -184,21 +251,8 @@ def main():
                    "max_density_error_last": st.max_density_error,
                    "neighbor_overflow": st.neighbor_overflow},
     }
-    gathers = {k: v for k, v in times.items() if k in KERNEL_BYTES}
-    dom = max(gathers, key=lambda k: gathers[k][0])
-    ms, cnt = gathers[dom]
-    avg_s = ms / cnt / 1e3
-    L = 4.0 * (nbar + 1.0)
-    bytes_launch = KERNEL_BYTES[dom](L) * n
-    achieved = bytes_launch / avg_s / 1e9
-    traffic = read_traffic(dom, "cfg2" if args.scale == 1.0 else f"scale{args.scale}")
-    out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": achieved,
-                       "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                       "traffic": traffic, "algorithmic_bytes_per_launch": bytes_launch,
-                       "avg_launch_ms": avg_s * 1e3, "launches": cnt}
-    out["kernels_ms_per_substep"] = {k: v[0] / args.steps for k, v in
-                                     sorted(times.items(), key=lambda kv: -kv[1][0])}
-    out["kernels_ms_per_substep"]["_sum"] = sum(v[0] for v in times.values()) / args.steps
+    out["roofline"], out["kernels_ms_per_substep"] = roofline(
+        times, n, nbar, args.steps, "cfg2" if args.scale == 1.0 else f"scale{args.scale}")
     if not args.no_extras:
         # Jacobi iterations and wall time to max density error <= 0.1% (cap 10),
         # plain and warm-started, from the same state (the state after the
