@@ -294,6 +294,21 @@ int sph_profile_reset(sph_ctx *ctx);
 const char *sph_profile_kernel_name(int kernel);
 int sph_profile_kernel_count(void);
 
+/* --- launch autotuner (no counterpart in the reference) ------------------
+ * Times `substeps` substeps for each candidate launch configuration from the
+ * context's current state (restored afterwards): 64/128/256 threads per block
+ * for every neighbour-gather kernel, and, with an obstacle, the push fused
+ * into the position update or as a separate pass.  Keeps the fastest choice
+ * per kernel.  Results do not depend on the choice. */
+#define SPH_TUNE_KERNELS 32
+typedef struct sph_tuning {
+  int split_collide;              /* obstacle push as a separate pass */
+  int block[SPH_TUNE_KERNELS];    /* threads per block, by sph_profile_kernel_name index */
+  double ms_per_substep_default;  /* event-timed kernel sum before tuning */
+  double ms_per_substep_tuned;    /* and with the chosen configuration */
+} sph_tuning;
+int sph_autotune(sph_ctx *ctx, int substeps, sph_tuning *result);
+
 /* --- learned lambda warm start ------------------------------------------
  * No counterpart in the reference: a 12 -> 64 -> 64 -> 1 SiLU network that
  * predicts, per particle, the accumulated multiplier of the constraint solve

@@ -58,6 +58,11 @@ class Scene(C.Structure):
                 ("size_z", C.c_double), ("obstacle", ObstacleDesc)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("split_collide", C.c_int), ("block", C.c_int * 32),
+                ("ms_per_substep_default", C.c_double), ("ms_per_substep_tuned", C.c_double)]
+
+
 class Stats(C.Structure):
     _fields_ = [("particles", C.c_int), ("iterations", C.c_int), ("converged", C.c_int),
                 ("max_density_error", C.c_float), ("neighbor_overflow", C.c_ulonglong),
@@ -143,6 +148,7 @@ def lib():
     L.sph_mlp_load.argtypes = [vp, C.c_char_p]
     L.sph_set_warm_start.argtypes = [vp, vp]
     L.sph_set_graph.argtypes = [vp, ip]
+    L.sph_autotune.argtypes = [vp, ip, C.POINTER(Tuning)]
     L.sph_create_batch.argtypes = [C.POINTER(Params), ip, C.POINTER(Scene), C.POINTER(vp)]
     L.sph_batch_scene_count.argtypes = [vp, C.POINTER(ip)]
     L.sph_batch_scene_errors.argtypes = [vp, fp]
@@ -290,6 +296,16 @@ class Context:
         e = C.c_double()
         check(lib().sph_compute_energy(self._h, C.byref(e)))
         return e.value
+
+    def autotune(self, substeps=3):
+        """Launch autotuner; returns {kernel name: threads per block}, split, timings."""
+        t = Tuning()
+        check(lib().sph_autotune(self._h, int(substeps), C.byref(t)))
+        names = kernel_names()
+        blocks = {names[k]: t.block[k] for k in range(len(names))}
+        return dict(block=blocks, split_collide=bool(t.split_collide),
+                    ms_per_substep_default=t.ms_per_substep_default,
+                    ms_per_substep_tuned=t.ms_per_substep_tuned)
 
     def residuals(self):
         """Max density error of each density pass of the last fused substep."""

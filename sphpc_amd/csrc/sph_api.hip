@@ -77,6 +77,14 @@ static void prof_collect(sph_ctx *c) {
   c->prof.pending.clear();
 }
 
+static void prof_zero(sph_ctx *c) {
+  prof_collect(c);
+  for (int k = 0; k < K_COUNT; ++k) {
+    c->prof.total_ms[k] = 0.0;
+    c->prof.launches[k] = 0;
+  }
+}
+
 static const char *kKernelNames[K_COUNT] = {
     "predict_hash", "hash", "gravity", "predict", "radix_upsweep", "radix_downsweep",
     "scan", "reorder", "build_lists", "density_lambda", "density", "lambda", "dp",
@@ -273,7 +281,7 @@ static int ctx_allocate(sph_ctx *c, int max_particles, sph_ctx **out) {
       {(void **)&c->list_cnt, sizeof(int) * cap},
       {(void **)&c->pw, f4},
       {(void **)&c->stats, sizeof(uint32_t) * STAT_WORDS},
-      {(void **)&c->pair_partial, sizeof(uint32_t) * blocks_for((uint32_t)cap)},
+      {(void **)&c->pair_partial, sizeof(uint32_t) * blocks_for((uint32_t)cap, kWave)},
       {(void **)&c->red, sizeof(double) * 4},
   };
   for (auto &a : allocs) {
@@ -1094,6 +1102,125 @@ extern "C" int sph_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
 // --------------------------------------------------------------------------
 // Timing
 // --------------------------------------------------------------------------
+// Launch autotuner.  Tries 64, 128 and 256 threads per block for every
+// neighbour-gather kernel and, with an obstacle, the fused versus separate
+// obstacle push, timing `substeps` substeps of each candidate with HIP events
+// from the same saved state; keeps the fastest choice per kernel and restores
+// the state.  Launch geometry does not change any result.
+static const int kTunable[] = {K_BUILD_LISTS, K_DENSITY_LAMBDA, K_DENSITY, K_LAMBDA, K_DP,
+                               K_DP_UPDATE, K_COLOR, K_SURFACE, K_XSPH, K_VORTICITY,
+                               K_CONFINE, K_WARM_LAMBDA};
+
+extern "C" int sph_autotune(sph_ctx *c, int substeps, sph_tuning *out) {
+  int rc = require_state(c);
+  if (rc) return rc;
+  if (substeps < 1) return arg_error("substeps must be >= 1");
+  const size_t n = (size_t)c->n;
+  if (!n) return SPH_OK;
+  float4 *sx = nullptr, *sv = nullptr;
+  float *sl = nullptr;
+  SPH_HIP(hipMalloc((void **)&sx, sizeof(float4) * n));
+  hipError_t e = hipMalloc((void **)&sv, sizeof(float4) * n);
+  if (e == hipSuccess && c->lam_prev) e = hipMalloc((void **)&sl, sizeof(float) * n);
+  auto release = [&]() {
+    if (sx) (void)hipFree(sx);
+    if (sv) (void)hipFree(sv);
+    if (sl) (void)hipFree(sl);
+  };
+  if (e != hipSuccess) {
+    release();
+    return hip_fail(e, "hipMalloc (autotune snapshot)");
+  }
+  auto copy_state = [&](bool save) -> hipError_t {
+    hipError_t r = hipMemcpyAsync(save ? sx : c->x_can, save ? c->x_can : sx, sizeof(float4) * n,
+                                  hipMemcpyDeviceToDevice, c->stream);
+    if (r == hipSuccess)
+      r = hipMemcpyAsync(save ? sv : c->v_can, save ? c->v_can : sv, sizeof(float4) * n,
+                         hipMemcpyDeviceToDevice, c->stream);
+    if (r == hipSuccess && sl)
+      r = hipMemcpyAsync(save ? sl : c->lam_prev, save ? c->lam_prev : sl, sizeof(float) * n,
+                         hipMemcpyDeviceToDevice, c->stream);
+    if (r == hipSuccess) r = hipStreamSynchronize(c->stream);
+    return r;
+  };
+  const long long substeps0 = c->substeps;
+  const bool prof0 = c->prof.on;
+  const auto tune0 = c->tune;
+  e = copy_state(true);
+  // one trial: restore, one untimed substep, `substeps` timed ones
+  auto trial = [&](double *per_kernel, double *total) -> int {
+    hipError_t r = copy_state(false);
+    if (r != hipSuccess) return hip_fail(r, "autotune restore");
+    c->prof.on = false;
+    int q = sph_step(c, 1);
+    if (q) return q;
+    prof_zero(c);
+    c->prof.on = true;
+    q = sph_step(c, substeps);
+    if (q) return q;
+    prof_collect(c);
+    *total = 0.0;
+    for (int k = 0; k < K_COUNT; ++k) {
+      per_kernel[k] = c->prof.launches[k] ? c->prof.total_ms[k] / substeps : 0.0;
+      *total += per_kernel[k];
+    }
+    c->prof.on = false;
+    return SPH_OK;
+  };
+  double best[K_COUNT], cur[K_COUNT], total = 0.0, before = 0.0;
+  uint16_t best_bs[K_COUNT] = {};
+  for (int k = 0; k < K_COUNT; ++k) best[k] = 1e300;
+  const uint16_t sizes[3] = {64, 128, 256};
+  for (int b = 0; b < 3 && e == hipSuccess && !rc; ++b) {
+    for (int k : kTunable) c->tune.block[k] = sizes[b];
+    graph_reset(c);
+    rc = trial(cur, &total);
+    if (rc) break;
+    if (sizes[b] == 256) before = total;
+    for (int k : kTunable)
+      if (cur[k] > 0.0 && cur[k] < best[k]) {
+        best[k] = cur[k];
+        best_bs[k] = sizes[b];
+      }
+  }
+  bool split_best = c->tune.split_collide;
+  double after = 0.0;
+  if (!rc && e == hipSuccess) {
+    for (int k : kTunable) c->tune.block[k] = best_bs[k] ? best_bs[k] : (uint16_t)kBlock;
+    const bool obstacle = c->nscenes ? c->batch_sdf != nullptr : c->sdf.data != nullptr;
+    double t_split[2] = {1e300, 1e300};
+    for (int sp = 0; sp < (obstacle ? 2 : 1) && !rc; ++sp) {
+      c->tune.split_collide = obstacle ? (sp == 0) : tune0.split_collide;
+      graph_reset(c);
+      rc = trial(cur, &total);
+      t_split[sp] = total;
+    }
+    if (obstacle && !rc) split_best = t_split[0] <= t_split[1];
+    after = obstacle ? std::min(t_split[0], t_split[1]) : t_split[0];
+    c->tune.split_collide = split_best;
+  }
+  if (rc || e != hipSuccess) c->tune = tune0;
+  const hipError_t er = copy_state(false);
+  release();
+  prof_zero(c);
+  c->prof.on = prof0;
+  c->substeps = substeps0;
+  c->lists_valid = c->sorted_valid = false;
+  graph_reset(c);
+  if (rc) return rc;
+  if (e != hipSuccess) return hip_fail(e, "autotune snapshot");
+  if (er != hipSuccess) return hip_fail(er, "autotune restore");
+  if (out) {
+    memset(out, 0, sizeof(*out));
+    out->split_collide = split_best;
+    for (int k = 0; k < K_COUNT && k < SPH_TUNE_KERNELS; ++k)
+      out->block[k] = c->tune.block[k] ? c->tune.block[k] : kBlock;
+    out->ms_per_substep_default = before;
+    out->ms_per_substep_tuned = after;
+  }
+  return SPH_OK;
+}
+
 extern "C" int sph_profile_enable(sph_ctx *c, int enable) {
   if (!c) return arg_error("context is NULL");
   if (!enable) prof_collect(c);

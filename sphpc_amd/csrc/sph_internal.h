@@ -182,6 +182,7 @@ struct sph_ctx {
   // launch choices (sph_autotune)
   struct {
     bool split_collide = true;  // obstacle push as its own pass after k_dp_update
+    uint16_t block[sph::K_COUNT] = {};  // threads per block of gather kernels (0: 256)
   } tune;
   // whole-substep HIP graph (sph_set_graph)
   bool graph_on = false;

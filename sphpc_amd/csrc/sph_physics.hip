@@ -28,7 +28,7 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
 }
 
 __device__ __forceinline__ uint32_t particle_index(uint32_t n_blocks) {
-  return xcd_block(blockIdx.x, n_blocks) * kBlock + threadIdx.x;
+  return xcd_block(blockIdx.x, n_blocks) * blockDim.x + threadIdx.x;
 }
 
 __device__ __forceinline__ const uint32_t *list_ptr(const uint32_t *list, uint32_t s) {
@@ -176,8 +176,7 @@ __device__ __forceinline__ void block_sum_to(uint32_t *partial, uint32_t v) {
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t s = 0;
-#pragma unroll
-    for (int w = 0; w < kBlock / kWave; ++w) s += wsum[w];
+    for (uint32_t w = 0; w < blockDim.x / kWave; ++w) s += wsum[w];
     partial[blockIdx.x] = s;
   }
 }
@@ -300,7 +299,7 @@ __global__ __launch_bounds__(kBlock) void k_build_lists(
     const uint32_t *__restrict__ key_s, const uint32_t *__restrict__ cstart, DevParams P,
     uint32_t *__restrict__ list, int *__restrict__ cnt, uint32_t *__restrict__ stats,
     uint32_t *__restrict__ pair_partial, BatchDev B) {
-  __shared__ uint16_t sbuf[kMaxNb][kBlock];
+  extern __shared__ uint16_t sbuf[];  // [kMaxNb][blockDim.x]
   const uint32_t s = particle_index(nblocks);
   int c = 0;
   uint32_t dropped = 0;
@@ -361,7 +360,7 @@ __global__ __launch_bounds__(kBlock) void k_build_lists(
                 if (direct)
                   out[(size_t)c * kWave] = j;
                 else
-                  sbuf[c][threadIdx.x] = (uint16_t)(tag | (j - lb));
+                  sbuf[c * blockDim.x + threadIdx.x] = (uint16_t)(tag | (j - lb));
                 ++c;
               } else {
                 ++dropped;
@@ -379,7 +378,7 @@ __global__ __launch_bounds__(kBlock) void k_build_lists(
   for (int o = 32; o >= 1; o >>= 1) cmax = max(cmax, __shfl_xor(cmax, o, 64));
   for (int k = 0; k < cmax; ++k) {
     if (k < c && !direct) {
-      const uint32_t e = sbuf[k][threadIdx.x];
+      const uint32_t e = sbuf[k * blockDim.x + threadIdx.x];
       const uint32_t l = e >> 14;
       const uint32_t b = l == 0 ? base[0] : (l == 1 ? base[1] : base[2]);
       out[(size_t)k * kWave] = b + (e & (kLayerSpan - 1u));
@@ -1096,6 +1095,12 @@ __global__ void k_zero_f64(double *p) { *p = 0.0; }
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
+// Threads per block of a gather kernel: the autotuner's choice or kBlock.
+static inline uint32_t tuned_block(const sph_ctx *c, int kernel) {
+  const uint32_t b = c->tune.block[kernel];
+  return b ? b : (uint32_t)kBlock;
+}
+
 #define LAUNCH_TIMED(ctx, kid, grid, ...)                               \
   do {                                                                  \
     hipEvent_t ev_ = nullptr;                                           \
@@ -1177,15 +1182,16 @@ int launch_neighbor_search(sph_ctx *c, bool fused) {
                  k_reorder<<<blocks_for(n), kBlock, 0, s>>>(n, c->id_s, c->xs_can, c->x_can,
                                                             c->pos, c->xold));
     SPH_HIP(hipGetLastError());
-    const uint32_t nb = blocks_for(n);
+    const uint32_t bs = tuned_block(c, K_BUILD_LISTS);
+  const uint32_t nb = blocks_for(n, bs);
     if (c->nscenes)
       LAUNCH_TIMED(c, K_BUILD_LISTS, 0,
-                   k_build_lists<true><<<nb, kBlock, 0, s>>>(
+                   k_build_lists<true><<<nb, bs, kMaxNb * bs * sizeof(uint16_t), s>>>(
                        n, nb, c->pos, c->key_s, c->cell_start, c->dev, c->list, c->list_cnt,
                        c->stats, c->pair_partial, B));
     else
       LAUNCH_TIMED(c, K_BUILD_LISTS, 0,
-                   k_build_lists<false><<<nb, kBlock, 0, s>>>(
+                   k_build_lists<false><<<nb, bs, kMaxNb * bs * sizeof(uint16_t), s>>>(
                        n, nb, c->pos, c->key_s, c->cell_start, c->dev, c->list, c->list_cnt,
                        c->stats, c->pair_partial, B));
     c->pair_blocks = nb;
@@ -1209,26 +1215,27 @@ static float stiffness_scale(const sph_ctx *c, int substep) {
 int launch_density(sph_ctx *c, int mode, int err_slot) {
   const uint32_t n = c->n;
   if (!n) return SPH_OK;
-  const uint32_t nb = blocks_for(n);
+  const uint32_t bs = tuned_block(c, K_DENSITY_LAMBDA);
+  const uint32_t nb = blocks_for(n, bs);
   hipStream_t s = c->stream;
   if (mode == 0)
     LAUNCH_TIMED(c, K_DENSITY_LAMBDA, 0,
-                 k_density_lambda<0><<<nb, kBlock, 0, s>>>(n, nb, c->pos, c->list, c->list_cnt,
+                 k_density_lambda<0><<<nb, bs, 0, s>>>(n, nb, c->pos, c->list, c->list_cnt,
                                                            c->dev, c->rho, c->stats, err_slot,
                                                            c->P.density_tolerance > 0, 0));
   else if (mode == 1)
     LAUNCH_TIMED(c, K_DENSITY, 0,
-                 k_density_lambda<1><<<nb, kBlock, 0, s>>>(n, nb, c->pos, c->list, c->list_cnt,
+                 k_density_lambda<1><<<nb, bs, 0, s>>>(n, nb, c->pos, c->list, c->list_cnt,
                                                            c->dev, c->rho, c->stats, err_slot, 0,
                                                            0));
   else if (mode == 3)  // final density of a capped tolerance loop: rho also into w
     LAUNCH_TIMED(c, K_DENSITY, 0,
-                 k_density_lambda<1><<<nb, kBlock, 0, s>>>(n, nb, c->pos, c->list, c->list_cnt,
+                 k_density_lambda<1><<<nb, bs, 0, s>>>(n, nb, c->pos, c->list, c->list_cnt,
                                                            c->dev, c->rho, c->stats, err_slot, 1,
                                                            1));
   else
     LAUNCH_TIMED(c, K_LAMBDA, 0,
-                 k_density_lambda<2><<<nb, kBlock, 0, s>>>(n, nb, c->pos, c->list, c->list_cnt,
+                 k_density_lambda<2><<<nb, bs, 0, s>>>(n, nb, c->pos, c->list, c->list_cnt,
                                                            c->dev, c->rho, c->stats, -1, 0, 0));
   SPH_HIP(hipGetLastError());
   return SPH_OK;
@@ -1237,9 +1244,10 @@ int launch_density(sph_ctx *c, int mode, int err_slot) {
 int launch_dp(sph_ctx *c, int substep) {
   const uint32_t n = c->n;
   if (!n) return SPH_OK;
-  const uint32_t nb = blocks_for(n);
+  const uint32_t bs = tuned_block(c, K_DP);
+  const uint32_t nb = blocks_for(n, bs);
   LAUNCH_TIMED(c, K_DP, 0,
-               k_dp<<<nb, kBlock, 0, c->stream>>>(n, nb, c->pos, c->list, c->list_cnt, c->dev,
+               k_dp<<<nb, bs, 0, c->stream>>>(n, nb, c->pos, c->list, c->list_cnt, c->dev,
                                                   stiffness_scale(c, substep), c->dpbuf));
   SPH_HIP(hipGetLastError());
   return SPH_OK;
@@ -1267,7 +1275,8 @@ static float stiffness_unitless(const sph_ctx *c, int substep) {
 int launch_dp_update(sph_ctx *c, int substep, int err_slot, bool accumulate) {
   const uint32_t n = c->n;
   if (!n) return SPH_OK;
-  const uint32_t nb = blocks_for(n);
+  const uint32_t bs = tuned_block(c, K_DP_UPDATE);
+  const uint32_t nb = blocks_for(n, bs);
   float4 *out = (c->pos == c->pos_a) ? c->pos_b : c->pos_a;
   float *acc = accumulate ? c->lam_acc : nullptr;
   const float scale = stiffness_scale(c, substep), stiff = stiffness_unitless(c, substep);
@@ -1277,7 +1286,7 @@ int launch_dp_update(sph_ctx *c, int substep, int err_slot, bool accumulate) {
   const bool split = obstacle && c->tune.split_collide;
 #define DP_UPDATE(COL, BAT)                                                              \
   LAUNCH_TIMED(c, K_DP_UPDATE, 0,                                                        \
-               (k_dp_update<COL, BAT><<<nb, kBlock, 0, c->stream>>>(                     \
+               (k_dp_update<COL, BAT><<<nb, bs, 0, c->stream>>>(                     \
                    n, nb, c->pos, out, c->rho, c->list, c->list_cnt, c->dev, scale, c->sdf, \
                    c->stats, err_slot, tol, acc, stiff, B)))
   if (c->nscenes) {
@@ -1305,12 +1314,13 @@ int launch_dp_update(sph_ctx *c, int substep, int err_slot, bool accumulate) {
 int launch_warm_lambda(sph_ctx *c, bool log) {
   const uint32_t n = c->n;
   if (!n) return SPH_OK;
-  const uint32_t nb = blocks_for(n);
+  const uint32_t bs = tuned_block(c, K_WARM_LAMBDA);
+  const uint32_t nb = blocks_for(n, bs);
   const BatchDev B = c->batch();
   float2 *plain = reinterpret_cast<float2 *>(c->dpbuf);
 #define WARM(LOG, BAT)                                                                      \
   LAUNCH_TIMED(c, K_WARM_LAMBDA, 0,                                                         \
-               (k_warm_lambda<LOG, BAT><<<nb, kBlock, 0, c->stream>>>(                       \
+               (k_warm_lambda<LOG, BAT><<<nb, bs, 0, c->stream>>>(                       \
                    n, nb, c->pos, c->xold, c->list, c->list_cnt, c->dev, c->rho, c->stats,   \
                    c->id_s, c->lam_prev, c->sdf, LOG ? nullptr : c->warm_net, c->lam_acc,   \
                    LOG ? c->feat : nullptr, LOG ? nullptr : plain, B)))
@@ -1383,15 +1393,16 @@ int launch_color(sph_ctx *c, bool with_velocity) {
   const uint32_t n = c->n;
   if (with_velocity) c->vel = c->vel_a;
   if (!n) return SPH_OK;
-  const uint32_t nb = blocks_for(n);
+  const uint32_t bs = tuned_block(c, K_COLOR);
+  const uint32_t nb = blocks_for(n, bs);
   if (with_velocity)
     LAUNCH_TIMED(c, K_COLOR, 0,
-                 k_color<true><<<nb, kBlock, 0, c->stream>>>(n, nb, c->pos, c->xold, c->list,
+                 k_color<true><<<nb, bs, 0, c->stream>>>(n, nb, c->pos, c->xold, c->list,
                                                              c->list_cnt, c->dev, c->vel,
                                                              c->aux));
   else
     LAUNCH_TIMED(c, K_COLOR, 0,
-                 k_color<false><<<nb, kBlock, 0, c->stream>>>(n, nb, c->pos, c->xold, c->list,
+                 k_color<false><<<nb, bs, 0, c->stream>>>(n, nb, c->pos, c->xold, c->list,
                                                               c->list_cnt, c->dev, c->vel,
                                                               c->aux));
   SPH_HIP(hipGetLastError());
@@ -1401,9 +1412,10 @@ int launch_color(sph_ctx *c, bool with_velocity) {
 int launch_surface(sph_ctx *c) {
   const uint32_t n = c->n;
   if (!n) return SPH_OK;
-  const uint32_t nb = blocks_for(n);
+  const uint32_t bs = tuned_block(c, K_SURFACE);
+  const uint32_t nb = blocks_for(n, bs);
   LAUNCH_TIMED(c, K_SURFACE, 0,
-               k_surface<<<nb, kBlock, 0, c->stream>>>(n, nb, c->pos, c->aux, c->list,
+               k_surface<<<nb, bs, 0, c->stream>>>(n, nb, c->pos, c->aux, c->list,
                                                        c->list_cnt, c->dev, c->vel));
   SPH_HIP(hipGetLastError());
   return SPH_OK;
@@ -1413,9 +1425,10 @@ int launch_xsph(sph_ctx *c) {
   const uint32_t n = c->n;
   float4 *out = (c->vel == c->vel_a) ? c->vel_b : c->vel_a;
   if (n) {
-    const uint32_t nb = blocks_for(n);
+    const uint32_t bs = tuned_block(c, K_XSPH);
+  const uint32_t nb = blocks_for(n, bs);
     LAUNCH_TIMED(c, K_XSPH, 0,
-                 k_xsph<<<nb, kBlock, 0, c->stream>>>(n, nb, c->pos, c->vel, c->list,
+                 k_xsph<<<nb, bs, 0, c->stream>>>(n, nb, c->pos, c->vel, c->list,
                                                       c->list_cnt, c->dev, out));
     SPH_HIP(hipGetLastError());
   }
@@ -1426,9 +1439,10 @@ int launch_xsph(sph_ctx *c) {
 int launch_vorticity(sph_ctx *c) {
   const uint32_t n = c->n;
   if (!n) return SPH_OK;
-  const uint32_t nb = blocks_for(n);
+  const uint32_t bs = tuned_block(c, K_VORTICITY);
+  const uint32_t nb = blocks_for(n, bs);
   LAUNCH_TIMED(c, K_VORTICITY, 0,
-               k_vorticity<<<nb, kBlock, 0, c->stream>>>(n, nb, c->pos, c->vel, c->list,
+               k_vorticity<<<nb, bs, 0, c->stream>>>(n, nb, c->pos, c->vel, c->list,
                                                          c->list_cnt, c->dev, c->aux, c->pw));
   SPH_HIP(hipGetLastError());
   return SPH_OK;
@@ -1438,15 +1452,16 @@ int launch_confine(sph_ctx *c, bool commit) {
   const uint32_t n = c->n;
   float4 *out = (c->vel == c->vel_a) ? c->vel_b : c->vel_a;
   if (n) {
-    const uint32_t nb = blocks_for(n);
+    const uint32_t bs = tuned_block(c, K_CONFINE);
+  const uint32_t nb = blocks_for(n, bs);
     if (commit)
       LAUNCH_TIMED(c, K_CONFINE, 0,
-                   k_confine<true><<<nb, kBlock, 0, c->stream>>>(
+                   k_confine<true><<<nb, bs, 0, c->stream>>>(
                        n, nb, c->pw, c->aux, c->list, c->list_cnt, c->dev, c->vel, out,
                        c->id_s, c->x_can, c->v_can));
     else
       LAUNCH_TIMED(c, K_CONFINE, 0,
-                   k_confine<false><<<nb, kBlock, 0, c->stream>>>(
+                   k_confine<false><<<nb, bs, 0, c->stream>>>(
                        n, nb, c->pw, c->aux, c->list, c->list_cnt, c->dev, c->vel, out,
                        c->id_s, c->x_can, c->v_can));
     SPH_HIP(hipGetLastError());

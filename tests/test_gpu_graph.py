@@ -82,3 +82,19 @@ def test_graph_replay_with_warm_start():
         graph.step(1)
         _same(eager, graph, step)
         assert np.array_equal(eager.field(sa.FIELD_LAMBDA_SUM), graph.field(sa.FIELD_LAMBDA_SUM))
+
+
+def test_autotune_keeps_state_and_results():
+    """The tuner restores the state it started from, and substeps with the
+    tuned launch geometry are bit-identical to the default geometry."""
+    sc, (ref, tuned) = _pair(1e-3)
+    p0, v0 = tuned.download()
+    t = tuned.autotune(2)
+    p1, v1 = tuned.download()
+    assert np.array_equal(p0, p1) and np.array_equal(v0, v1)
+    assert all(b in (64, 128, 256) for b in t["block"].values())
+    assert t["ms_per_substep_tuned"] > 0
+    for step in range(3):
+        ref.step(1)
+        tuned.step(1)
+        _same(ref, tuned, step)
