@@ -353,6 +353,10 @@ int sph_mlp_fit_normalization(sph_mlp *net, const float *x_dev, const float *t_d
  * the network.  Training clips its targets at the same bound. */
 int sph_mlp_set_output_clip(sph_mlp *net, float max_target);
 float sph_mlp_get_output_clip(sph_mlp *net);
+/* The solver applies scale * Lambda_pred (default 1; a damping fitted on held-out
+ * scenes by the trainer); saved with the network. */
+int sph_mlp_set_step_scale(sph_mlp *net, float scale);
+float sph_mlp_get_step_scale(sph_mlp *net);
 /* Full-batch gradient of 0.5/B sum (y_n - t_n)^2 in normalised target units
  * (host outputs; either may be NULL). */
 int sph_mlp_gradient(sph_mlp *net, const float *x_dev, const float *t_dev, size_t count,

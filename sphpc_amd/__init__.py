@@ -156,6 +156,9 @@ def lib():
     L.sph_mlp_set_output_clip.argtypes = [vp, C.c_float]
     L.sph_mlp_get_output_clip.argtypes = [vp]
     L.sph_mlp_get_output_clip.restype = C.c_float
+    L.sph_mlp_set_step_scale.argtypes = [vp, C.c_float]
+    L.sph_mlp_get_step_scale.argtypes = [vp]
+    L.sph_mlp_get_step_scale.restype = C.c_float
     L.sph_set_sample_logging.argtypes = [vp, ip]
     L.sph_warm_samples.argtypes = [vp, vp, vp, sz, sz, ip, C.POINTER(sz)]
     _lib = L
@@ -469,6 +472,12 @@ class Mlp:
 
     def output_clip(self):
         return lib().sph_mlp_get_output_clip(self._h)
+
+    def set_step_scale(self, scale):
+        check(lib().sph_mlp_set_step_scale(self._h, float(scale)))
+
+    def step_scale(self):
+        return lib().sph_mlp_get_step_scale(self._h)
 
     def fit_normalization(self, x_buf, t_buf, count):
         check(lib().sph_mlp_fit_normalization(self._h, x_buf.ptr, t_buf.ptr, int(count)))

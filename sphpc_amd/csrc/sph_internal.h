@@ -88,9 +88,10 @@ constexpr int kMlpW1 = 0, kMlpB1 = kMlpW1 + kMlpPad * kMlpH, kMlpW2 = kMlpB1 + k
               kMlpB2 = kMlpW2 + kMlpH * kMlpH, kMlpW3 = kMlpB2 + kMlpH,
               kMlpB3 = kMlpW3 + kMlpH, kMlpParams = kMlpB3 + 1;  // 5313
 // Normalisation words: feature means, reciprocal deviations, target shift/scale.
-// kNormYMax > 0 caps the network output (the clipping bound of its training targets).
+// kNormYMax > 0 caps the network output (the clipping bound of its training
+// targets); kNormStep > 0 scales the predicted multiplier (damping, 0 means 1).
 constexpr int kNormMu = 0, kNormIsd = 16, kNormYShift = 32, kNormYScale = 33, kNormYMax = 34,
-              kNormWords = 36;
+              kNormStep = 35, kNormWords = 36;
 
 // BF16 inference weights packed in MFMA operand order (layout in sph_mlp_dev.h).
 struct MlpPacked {

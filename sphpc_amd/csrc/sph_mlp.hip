@@ -521,6 +521,17 @@ extern "C" int sph_mlp_set_output_clip(sph_mlp *m, float max_target) {
 
 extern "C" float sph_mlp_get_output_clip(sph_mlp *m) { return m ? m->host_norm[kNormYMax] : 0.0f; }
 
+extern "C" int sph_mlp_set_step_scale(sph_mlp *m, float scale) {
+  if (!m) return mlp_arg("NULL argument");
+  if (!(scale > 0.0f)) return mlp_arg("scale must be positive");
+  m->host_norm[kNormStep] = scale;
+  return upload_norm(m);
+}
+
+extern "C" float sph_mlp_get_step_scale(sph_mlp *m) {
+  return (m && m->host_norm[kNormStep] > 0.0f) ? m->host_norm[kNormStep] : 1.0f;
+}
+
 extern "C" int sph_mlp_get_normalization(sph_mlp *m, float *mean, float *stddev,
                                          float *target_mean, float *target_std) {
   if (!m) return mlp_arg("NULL argument");

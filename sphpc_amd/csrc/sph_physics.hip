@@ -35,28 +35,40 @@ __device__ __forceinline__ const uint32_t *list_ptr(const uint32_t *list, uint32
   return list + (size_t)(s >> 6) * kListStride + (s & 63u);
 }
 
-// Visits the c neighbours of a particle in list order.  Each round issues the
-// kBatch dependent gathers of this batch, then the kBatch list loads of the
-// next batch, then the arithmetic: the list loads of batch k+1 are in flight
-// while batch k is gathered and used, so a batch costs one memory round trip
-// instead of two.  Padding slots load the particle itself (`self`, always a
-// valid index) and are skipped by the body.
+// Visits the c neighbours of a particle in list order, kBatch at a time: first
+// the batch's list loads, then its dependent gathers, then the arithmetic, so
+// every wave keeps kBatch loads in flight instead of one dependent chain per
+// neighbour.  With PREFETCH the list loads of batch k+1 are issued right after
+// the gathers of batch k, so a batch costs one memory round trip instead of
+// two; that pays for 16-byte gathers (-10% time per pass) and not for the
+// 32-byte (Pair4) ones, whose extra registers cost more occupancy than the
+// hidden latency gains (+30% measured).  Padding slots load the particle itself
+// (`self`, always a valid index) and are skipped by the body.
 constexpr int kBatch = 8;
 
-template <typename T, typename Load, typename Body>
+template <typename T, bool PREFETCH = true, typename Load, typename Body>
 __device__ __forceinline__ void for_neighbors(const uint32_t *L, int c, uint32_t self, Load load,
                                               Body body) {
   uint32_t jn[kBatch];
 #pragma unroll
   for (int b = 0; b < kBatch; ++b) jn[b] = (b < c) ? L[(size_t)b * kWave] : self;
   for (int k0 = 0; k0 < c; k0 += kBatch) {
+    if (!PREFETCH && k0 > 0) {
+#pragma unroll
+      for (int b = 0; b < kBatch; ++b) {
+        const int k = k0 + b;
+        jn[b] = (k < c) ? L[(size_t)k * kWave] : self;
+      }
+    }
     T q[kBatch];
 #pragma unroll
     for (int b = 0; b < kBatch; ++b) q[b] = load(jn[b]);
+    if (PREFETCH) {
 #pragma unroll
-    for (int b = 0; b < kBatch; ++b) {
-      const int k = k0 + kBatch + b;
-      jn[b] = (k < c) ? L[(size_t)k * kWave] : self;
+      for (int b = 0; b < kBatch; ++b) {
+        const int k = k0 + kBatch + b;
+        jn[b] = (k < c) ? L[(size_t)k * kWave] : self;
+      }
     }
 #pragma unroll
     for (int b = 0; b < kBatch; ++b)
@@ -654,7 +666,7 @@ __global__ __launch_bounds__(kBlock) void k_warm_lambda(
     float su3 = 0.0f, sgx = 0.0f, sgy = 0.0f, sgz = 0.0f, sc = 0.0f;
     float mx = 0.0f, my = 0.0f, mz = 0.0f, dv = 0.0f;
     auto load = [&](uint32_t j) { return Pair4{pos[j], xold[j]}; };
-    for_neighbors<Pair4>(list_ptr(list, s), c, s, load, [&](const Pair4 &qq) {
+    for_neighbors<Pair4, false>(list_ptr(list, s), c, s, load, [&](const Pair4 &qq) {
       const float4 &q = qq.a, &qo = qq.b;
       const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
       const float r2 = dx * dx + dy * dy + dz * dz;
@@ -705,9 +717,9 @@ __global__ __launch_bounds__(kBlock) void k_warm_lambda(
     }
   } else {
     const float y = mlp_wave(f, net);
-    const float ymax = net->norm[kNormYMax];
+    const float ymax = net->norm[kNormYMax], step = net->norm[kNormStep];
     const float ce = ymax > 0.0f ? fminf(fmaxf(y, 0.0f), ymax) : fmaxf(y, 0.0f);
-    lam = -ce / denom;
+    lam = -(step > 0.0f ? step : 1.0f) * ce / denom;
     if (active) plain[s] = make_float2(lam0, rho);  // for the plain fallback
   }
   if (active) {
@@ -841,7 +853,7 @@ __global__ __launch_bounds__(kBlock) void k_surface(uint32_t n, uint32_t nblocks
   const float hh = 0.5f * P.h;
   float fx = 0.0f, fy = 0.0f, fz = 0.0f;
   auto load = [&](uint32_t j) { return Pair4{pos[j], color[j]}; };
-  for_neighbors<Pair4>(L, c, s, load, [&](const Pair4 &qc) {
+  for_neighbors<Pair4, false>(L, c, s, load, [&](const Pair4 &qc) {
     const float4 &q = qc.a, &cq = qc.b;
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     const float r2c = fmaxf(dx * dx + dy * dy + dz * dz, P.rmin2);
@@ -883,7 +895,7 @@ __global__ __launch_bounds__(kBlock) void k_xsph(uint32_t n, uint32_t nblocks,
   const int c = cnt[s];
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
   auto load = [&](uint32_t j) { return Pair4{pos[j], vin[j]}; };
-  for_neighbors<Pair4>(L, c, s, load, [&](const Pair4 &qv) {
+  for_neighbors<Pair4, false>(L, c, s, load, [&](const Pair4 &qv) {
     const float4 &q = qv.a, &vq = qv.b;
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     const float u = P.h2 - (dx * dx + dy * dy * dz * dz);
@@ -913,7 +925,7 @@ __global__ __launch_bounds__(kBlock) void k_vorticity(uint32_t n, uint32_t nbloc
   const int c = cnt[s];
   float ox = 0.0f, oy = 0.0f, oz = 0.0f;
   auto load = [&](uint32_t j) { return Pair4{pos[j], vel[j]}; };
-  for_neighbors<Pair4>(L, c, s, load, [&](const Pair4 &qv) {
+  for_neighbors<Pair4, false>(L, c, s, load, [&](const Pair4 &qv) {
     const float4 &q = qv.a, &vq = qv.b;
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     const float r2c = fmaxf(dx * dx + dy * dy + dz * dz, P.rmin2);
