@@ -311,11 +311,12 @@ int sph_autotune(sph_ctx *ctx, int substeps, sph_tuning *result);
 
 /* --- learned lambda warm start ------------------------------------------
  * No counterpart in the reference: a 12 -> 64 -> 64 -> 1 SiLU network that
- * predicts, per particle, the accumulated multiplier of the constraint solve
- * (sum over iterations of stiffness(it) * lambda_it, particles.c:664,693-694)
- * from quantities of the first density pass.  The plain Jacobi loop stays the
- * acceptance test: with a tolerance, iterations continue until the density
- * error is met, so the prediction can only save iterations.
+ * predicts, per particle, how much the iterations after the first add to the
+ * accumulated multiplier of the constraint solve, Lambda = sum over
+ * iterations of stiffness(it) * lambda_it (particles.c:664,693-694), from
+ * quantities of the first density pass; the first step then applies that
+ * remainder at once.  The plain Jacobi loop stays the acceptance test: with a
+ * tolerance, iterations continue until the density error is met.
  *
  * Dataset rows are SPH_MLP_STRIDE floats, device memory, 16-byte aligned:
  *   0 neighbour count / 32           (NeighborBucket.count, neighbors.h:27)
@@ -327,9 +328,12 @@ int sph_autotune(sph_ctx *ctx, int substeps, sph_tuning *result);
  *   6-8 normalised SDF gradient (0 outside the grid)
  *   9-11 first density moment sum_j m W_ij (x_j - x_i) / (h rho_i)
  *   12 |grad C|^2 + eps (not a network input; converts targets)
- *   13-15 unused
- * The target is C_eff = -Lambda * (|grad C|^2 + eps) (dimensionless); the
- * solver applies Lambda_pred = -max(C_eff_pred, 0) / (|grad C|^2 + eps). */
+ *   13 lambda_0, the plain first-iteration lambda (not a network input)
+ * The target is the remainder R = -(Lambda - lambda_0) (|grad C|^2 + eps)
+ * (dimensionless, >= 0): what iterations after the first add to the
+ * multiplier.  The warm-started first step uses lambda_0 - s max(R_pred, 0) /
+ * (|grad C|^2 + eps), s the network's step scale; with R_pred = 0 it is the
+ * plain first iteration. */
 #define SPH_MLP_FEATURES 12
 #define SPH_MLP_STRIDE 16
 #define SPH_MLP_PARAMS 5313

@@ -640,8 +640,10 @@ __device__ __forceinline__ void sdf_features(float x, float y, float z, const Sd
 // First pass of a warm-started or logged solve.  ComputeDensities and
 // ComputeLambda as in k_density_lambda, plus the network features, in one
 // gather of (x*, x) per neighbour.  LOG: writes the feature rows and the
-// plain lambda.  Otherwise runs the network on MFMA (BF16, FP32 accumulate)
-// and writes Lambda_pred = -max(C_eff, 0) / (|grad C|^2 + eps) as lambda.
+// plain lambda.  Otherwise runs the network on MFMA (BF16, FP32 accumulate),
+// whose output R estimates what the later iterations of a converged solve add
+// to the multiplier, and writes lambda_0 - scale * max(R, 0) / (|grad C|^2 + eps):
+// the plain first step plus the predicted remainder.
 // Either way the particle's accumulated multiplier restarts at zero.  Every
 // lane of a wave reaches the network (no early return).
 template <bool LOG, bool BATCH>
@@ -719,7 +721,7 @@ __global__ __launch_bounds__(kBlock) void k_warm_lambda(
     const float y = mlp_wave(f, net);
     const float ymax = net->norm[kNormYMax], step = net->norm[kNormStep];
     const float ce = ymax > 0.0f ? fminf(fmaxf(y, 0.0f), ymax) : fmaxf(y, 0.0f);
-    lam = -(step > 0.0f ? step : 1.0f) * ce / denom;
+    lam = lam0 - (step > 0.0f ? step : 1.0f) * ce / denom;
     if (active) plain[s] = make_float2(lam0, rho);  // for the plain fallback
   }
   if (active) {
@@ -766,7 +768,8 @@ __global__ __launch_bounds__(kBlock) void k_scatter_lambda(uint32_t n,
 }
 
 // Dataset rows of a logged substep: every `every`-th sorted particle's
-// feature row and its target C_eff = -Lambda * (|grad C|^2 + eps).
+// feature row and its target R = -(Lambda - lambda_0) * (|grad C|^2 + eps), the
+// part of the accumulated multiplier after the first iteration.
 __global__ __launch_bounds__(kBlock) void k_warm_samples(uint32_t count, uint32_t every,
                                                          const float *__restrict__ feat,
                                                          const float *__restrict__ lam_acc,
@@ -782,7 +785,7 @@ __global__ __launch_bounds__(kBlock) void k_warm_samples(uint32_t count, uint32_
   dst[1] = src[1];
   dst[2] = src[2];
   dst[3] = r3;
-  t[k] = -lam_acc[s] * r3.x;
+  t[k] = -(lam_acc[s] - r3.y) * r3.x;
 }
 
 // ---------------------------------------------------------------------------

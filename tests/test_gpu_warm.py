@@ -112,14 +112,18 @@ def test_logged_features_and_targets_match_oracle(with_obstacle):
     dens, lam = np.zeros(n), np.zeros(n)
     dpx, dpy, dpz = np.zeros(n), np.zeros(n), np.zeros(n)
     acc = np.zeros(n)
+    lam_first = None
     for it in range(sc.iters):
         L.orc_compute_densities(n, xs, ys, zs, nbr, cnt, C.byref(o), dens)
         L.orc_compute_lambda(n, xs, ys, zs, dens, nbr, cnt, C.byref(o), lam)
+        if lam_first is None:
+            lam_first = lam.copy()
         acc += L.orc_stiffness(C.byref(o), it) * lam
         L.orc_update_dps(n, xs, ys, zs, lam, nbr, cnt, C.byref(o), it, dpx, dpy, dpz)
         L.orc_update_position_stars(n, xs, ys, zs, dpx, dpy, dpz, C.byref(o),
                                     C.byref(ob) if ob else None)
-    t_ref = -acc * ref[:, 12]
+    t_ref = -(acc - lam_first) * ref[:, 12]  # remainder after the first iteration
+    assert rel_err(by_id[:, 13], lam_first) < 1e-4
     e = rel_err(t_id, t_ref)
     lam_sum = ctx.field(sa.FIELD_LAMBDA_SUM)
     e_sum = rel_err(lam_sum, acc)
@@ -172,9 +176,9 @@ def test_fused_inference_matches_standalone():
     net.infer_bf16(xb, n, yb)
     y = yb.to_array(np.float32, n).astype(np.float64)
     expect = np.zeros(n)
-    expect[ids] = -np.maximum(y, 0.0) / rows[:, 12]
+    expect[ids] = rows[:, 13] - np.maximum(y, 0.0) / rows[:, 12]
     # same state again, warm-started with one iteration: the accumulated
-    # multiplier is then exactly the network's prediction
+    # multiplier is then exactly the warm first step's lambda
     ctx.set_sample_logging(False)
     ctx.set_warm_start(net)
     ctx.upload(pos, vel)
@@ -271,41 +275,68 @@ def test_trained_network_saves_iterations():
             wm["mean_final_error"] < 0.9 * pl["mean_final_error"]), cmp
 
 
+def _face_net(amplitude):
+    """A network whose output is ~amplitude * relu(-m_x): large only for the
+    particles on the +x face of a block (x density moment < 0), zero inside."""
+    net = sa.Mlp(seed=1)
+    theta = np.zeros(sa.MLP_PARAMS, np.float32)
+    theta[9 * 64 + 0] = -1.0             # W1[9][0]: hidden 1 unit 0 = silu(-m_x / 0.05)
+    theta[1088 + 0 * 64 + 0] = 1.0       # W2[0][0]
+    theta[5248 + 0] = float(amplitude)   # w3[0]
+    net.set_weights(theta)
+    mean = np.zeros(12, np.float32)
+    std = np.ones(12, np.float32)
+    std[9] = 0.05
+    net.set_normalization(mean, std, 0.0, 1.0)
+    return net
+
+
 def test_bad_prediction_falls_back_to_plain():
-    """An under-dense block in the corner of the box: the plain lambda is zero
-    everywhere, so the plain solve leaves the positions alone.  A network that
-    predicts C_eff = 2 for every particle pushes particles into the walls,
-    where the clamp piles them up; the density error after that step exceeds
-    the error before it, so the solver must revert the step and finish
-    exactly like the plain solve."""
-    s = 0.02
-    sc = Scene(s, (2.0, 1.0, 1.0), iters=6, lambda_eps=600.0)
-    pos, vel = block_scene((16, 16, 16), s * 1.25, (0.0, 0.0, 0.0), jitter=0.01, seed=9)
+    """A prediction that is large only on one face of a block pushes the
+    layer behind that face into the interior: the density error after the
+    step exceeds the error before it, so the solver must revert the step,
+    report it, and continue from the unmodified positions."""
+    sc, pos, vel = _scene(iters=6)
     n = len(pos)
     plain = sc.context(n)
     plain.upload(pos, vel)
     plain.step(1)
     r_plain = plain.residuals()
-    net = sa.Mlp(seed=1)
-    theta = net.weights()
-    theta[:] = 0.0
-    theta[-1] = 2.0  # b3: C_eff = 2 for every particle
-    net.set_weights(theta)
     warm = sc.context(n)
     warm.upload(pos, vel)
-    warm.set_warm_start(net)
+    warm.set_warm_start(_face_net(40.0))
     warm.step(1)
     st = warm.stats()
     r_warm = warm.residuals()
     record("warm_fallback", fallback=st.warm_fallback, plain_final=float(r_plain[-1]),
-           warm_final=float(r_warm[-1]))
-    assert np.all(r_plain == 0.0)             # under-dense: nothing for the plain solve to do
+           warm_final=float(r_warm[-1]), e0=float(r_warm[0]))
     assert st.warm_fallback == 1
-    assert len(r_warm) == 6 and r_warm[0] == 0.0 and r_warm[1] == 0.0
-    pw, vw = warm.download()
-    pp, vp = plain.download()
-    assert np.max(np.abs(pw - pp)) <= 1e-6
-    assert np.max(np.abs(vw - vp)) <= 1e-4
+    assert len(r_plain) == len(r_warm) == 6
+    assert abs(r_warm[0] - r_plain[0]) <= 1e-6   # same state before any update
+    assert r_warm[1] == r_warm[0]                # slot of the reverted step: pre-step error
+    assert r_warm[-1] <= 2.0 * r_plain[-1] + 1e-4
+
+
+def test_harmless_prediction_is_kept():
+    """A zero remainder makes the warm first step the plain first step: no
+    fallback, and the substep equals the plain one."""
+    sc, pos, vel = _scene(iters=4)
+    n = len(pos)
+    plain = sc.context(n)
+    plain.upload(pos, vel)
+    plain.step(1)
+    warm = sc.context(n)
+    warm.upload(pos, vel)
+    warm.set_warm_start(_face_net(0.0))
+    warm.step(1)
+    r_plain = plain.residuals()
+    assert r_plain[1] <= r_plain[0]  # the plain first step lowers the error here
+    assert warm.stats().warm_fallback == 0
+    pw, _ = warm.download()
+    pp, _ = plain.download()
+    e = float(np.max(np.abs(pw - pp)))
+    record("warm_zero_remainder", max_abs_dx=e)
+    assert e <= 1e-6
 
 
 def test_residuals_report_every_density_pass():
