@@ -141,6 +141,20 @@ def roofline(times, n, nbar, steps, tag):
     return roof, per
 
 
+GATHER_KERNELS = ("build_lists", "density_lambda", "dp_update", "color", "surface", "xsph",
+                  "vorticity", "confine")
+
+
+def compact_tuning(t):
+    if t is None:
+        return None
+    return {"ms_per_substep_default": t["ms_per_substep_default"],
+            "ms_per_substep_tuned": t["ms_per_substep_tuned"],
+            "split_collide": t["split_collide"],
+            "threads_per_block": {k: t["block"][k] for k in GATHER_KERNELS},
+            "gathers_in_flight": {k: t["depth"][k] for k in GATHER_KERNELS if k != "build_lists"}}
+
+
 def bench_batch(args):
     """Batched scenes: `--scenes` independent scenes in one context, one launch
     per stage for all of them (sphpc_amd/workloads.py)."""
@@ -153,8 +167,11 @@ def bench_batch(args):
                           vorticity_epsilon=0.01, vorticity_model=1, solve_iterations=4)
     ctx = sa.BatchContext(p, scenes)
     ctx.upload(pos)
-    ctx.set_graph(not args.no_graph)
     ctx.step(args.warmup)
+    ctx.synchronize()
+    tuning = None if args.no_autotune else ctx.autotune(3)
+    ctx.set_graph(not args.no_graph)
+    ctx.step(1)
     ctx.synchronize()
     t0 = time.perf_counter()
     ctx.step(args.steps)
@@ -178,6 +195,7 @@ def bench_batch(args):
                                "4 Jacobi iterations, surface tension + XSPH + vorticity",
                    "particles": n, "scenes": args.scenes, "min_scene": int(min(counts)),
                    "max_scene": int(max(counts)), "graph": not args.no_graph,
+                   "autotune": compact_tuning(tuning),
                    "mean_neighbors": nbar,
                    "scene_max_density_error": {"mean": float(np.mean(errs)),
                                                "median": float(np.median(errs)),
@@ -208,6 +226,8 @@ def main():
     ap.add_argument("--scenes", type=int, default=512)
     ap.add_argument("--no-graph", action="store_true",
                     help="eager launches instead of a captured graph in the timed region")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="default launch configuration instead of the autotuner's")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run two 1 GiB device copies (counter calibration)")
     args = ap.parse_args()
@@ -239,6 +259,7 @@ def main():
 
     ctx.step(args.warmup)
     ctx.synchronize()
+    tuning = None if args.no_autotune else ctx.autotune(3)
     start = ctx.download()
     # Whole-job throughput: substeps replayed from a captured HIP graph (the
     # capture happens during the first, untimed call), state resident in HBM,
@@ -276,6 +297,7 @@ def main():
     out["roofline"], out["kernels_ms_per_substep"] = roofline(
         times, n, nbar, prof_steps, "cfg2" if args.scale == 1.0 else f"scale{args.scale}")
     out["config"]["graph"] = not args.no_graph
+    out["config"]["autotune"] = compact_tuning(tuning)
     if not args.no_extras:
         # Jacobi iterations and wall time to max density error <= 0.1% (cap 10),
         # plain and warm-started, from the same state (the state after the

@@ -297,13 +297,15 @@ int sph_profile_kernel_count(void);
 /* --- launch autotuner (no counterpart in the reference) ------------------
  * Times `substeps` substeps for each candidate launch configuration from the
  * context's current state (restored afterwards): 64/128/256 threads per block
- * for every neighbour-gather kernel, and, with an obstacle, the push fused
- * into the position update or as a separate pass.  Keeps the fastest choice
- * per kernel.  Results do not depend on the choice. */
+ * and 8/16/32 neighbour gathers in flight per wavefront for every
+ * neighbour-gather kernel, and, with an obstacle, the push fused into the
+ * position update or as a separate pass.  Keeps the fastest choice per
+ * kernel.  Results do not depend on the choice. */
 #define SPH_TUNE_KERNELS 32
 typedef struct sph_tuning {
   int split_collide;              /* obstacle push as a separate pass */
   int block[SPH_TUNE_KERNELS];    /* threads per block, by sph_profile_kernel_name index */
+  int depth[SPH_TUNE_KERNELS];    /* gathers in flight per wavefront, same index */
   double ms_per_substep_default;  /* event-timed kernel sum before tuning */
   double ms_per_substep_tuned;    /* and with the chosen configuration */
 } sph_tuning;

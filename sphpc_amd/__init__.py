@@ -59,7 +59,7 @@ class Scene(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("split_collide", C.c_int), ("block", C.c_int * 32),
+    _fields_ = [("split_collide", C.c_int), ("block", C.c_int * 32), ("depth", C.c_int * 32),
                 ("ms_per_substep_default", C.c_double), ("ms_per_substep_tuned", C.c_double)]
 
 
@@ -306,7 +306,8 @@ class Context:
         check(lib().sph_autotune(self._h, int(substeps), C.byref(t)))
         names = kernel_names()
         blocks = {names[k]: t.block[k] for k in range(len(names))}
-        return dict(block=blocks, split_collide=bool(t.split_collide),
+        depths = {names[k]: t.depth[k] for k in range(len(names))}
+        return dict(block=blocks, depth=depths, split_collide=bool(t.split_collide),
                     ms_per_substep_default=t.ms_per_substep_default,
                     ms_per_substep_tuned=t.ms_per_substep_tuned)
 

@@ -1010,9 +1010,9 @@ extern "C" int sph_download_neighbors(sph_ctx *c, uint32_t *out_ids, int *counts
   for (int s = 0; s < n; ++s) {
     const uint32_t id = ids[s];
     counts[id] = cnt[s];
-    const size_t base = (size_t)(s >> 6) * kListStride + (s & 63);
+    const size_t base = (size_t)(s >> 6) * kListStride + (size_t)(s & 63) * 4;
     for (int k = 0; k < cnt[s]; ++k)
-      out_ids[(size_t)id * kMaxNb + k] = ids[list[base + (size_t)k * kWave]];
+      out_ids[(size_t)id * kMaxNb + k] = ids[list[base + (size_t)(k >> 2) * kWave * 4 + (k & 3)]];
   }
   return SPH_OK;
 }
@@ -1102,11 +1102,12 @@ extern "C" int sph_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
 // --------------------------------------------------------------------------
 // Timing
 // --------------------------------------------------------------------------
-// Launch autotuner.  Tries 64, 128 and 256 threads per block for every
-// neighbour-gather kernel and, with an obstacle, the fused versus separate
-// obstacle push, timing `substeps` substeps of each candidate with HIP events
-// from the same saved state; keeps the fastest choice per kernel and restores
-// the state.  Launch geometry does not change any result.
+// Launch autotuner.  Tries 64, 128 and 256 threads per block and then 8, 16
+// and 32 gathers in flight per wave for every neighbour-gather kernel and,
+// with an obstacle, the fused versus separate obstacle push, timing
+// `substeps` substeps of each candidate with HIP events from the same saved
+// state; keeps the fastest choice per kernel and restores the state.  None of
+// the choices changes any result.
 static const int kTunable[] = {K_BUILD_LISTS, K_DENSITY_LAMBDA, K_DENSITY, K_LAMBDA, K_DP,
                                K_DP_UPDATE, K_COLOR, K_SURFACE, K_XSPH, K_VORTICITY,
                                K_CONFINE, K_WARM_LAMBDA};
@@ -1183,10 +1184,26 @@ extern "C" int sph_autotune(sph_ctx *c, int substeps, sph_tuning *out) {
         best_bs[k] = sizes[b];
       }
   }
+  // gathers in flight per wave, with the chosen block sizes
+  for (int k : kTunable) c->tune.block[k] = best_bs[k] ? best_bs[k] : (uint16_t)kBlock;
+  uint8_t best_depth[K_COUNT] = {};
+  for (int k = 0; k < K_COUNT; ++k) best[k] = 1e300;
+  const uint8_t depths[3] = {8, 16, 32};
+  for (int d = 0; d < 3 && e == hipSuccess && !rc; ++d) {
+    for (int k : kTunable) c->tune.depth[k] = depths[d];
+    graph_reset(c);
+    rc = trial(cur, &total);
+    if (rc) break;
+    for (int k : kTunable)
+      if (cur[k] > 0.0 && cur[k] < best[k]) {
+        best[k] = cur[k];
+        best_depth[k] = depths[d];
+      }
+  }
+  for (int k : kTunable) c->tune.depth[k] = best_depth[k] ? best_depth[k] : (uint8_t)16;
   bool split_best = c->tune.split_collide;
   double after = 0.0;
   if (!rc && e == hipSuccess) {
-    for (int k : kTunable) c->tune.block[k] = best_bs[k] ? best_bs[k] : (uint16_t)kBlock;
     const bool obstacle = c->nscenes ? c->batch_sdf != nullptr : c->sdf.data != nullptr;
     double t_split[2] = {1e300, 1e300};
     for (int sp = 0; sp < (obstacle ? 2 : 1) && !rc; ++sp) {
@@ -1213,8 +1230,10 @@ extern "C" int sph_autotune(sph_ctx *c, int substeps, sph_tuning *out) {
   if (out) {
     memset(out, 0, sizeof(*out));
     out->split_collide = split_best;
-    for (int k = 0; k < K_COUNT && k < SPH_TUNE_KERNELS; ++k)
+    for (int k = 0; k < K_COUNT && k < SPH_TUNE_KERNELS; ++k) {
       out->block[k] = c->tune.block[k] ? c->tune.block[k] : kBlock;
+      out->depth[k] = c->tune.depth[k] ? c->tune.depth[k] : 16;
+    }
     out->ms_per_substep_default = before;
     out->ms_per_substep_tuned = after;
   }

@@ -158,7 +158,7 @@ struct sph_ctx {
   float4 *pw = nullptr;   // (x*, |omega|) for the confinement gather
   float4 *dpbuf = nullptr;  // staged delta p
   float *rho = nullptr;
-  uint32_t *list = nullptr;     // sorted neighbour indices, slot-major per 64 particles
+  uint32_t *list = nullptr;     // sorted neighbour indices, quads of slots per 64 particles
   int *list_cnt = nullptr;
   uint32_t *stats = nullptr;  // sph::STAT_WORDS words
   uint32_t *pair_partial = nullptr;  // per-block list-length sums of the last search
@@ -184,6 +184,7 @@ struct sph_ctx {
   struct {
     bool split_collide = true;  // obstacle push as its own pass after k_dp_update
     uint16_t block[sph::K_COUNT] = {};  // threads per block of gather kernels (0: 256)
+    uint8_t depth[sph::K_COUNT] = {};   // gathers in flight per wave: 8, 16, 32 (0: 16)
   } tune;
   // whole-substep HIP graph (sph_set_graph)
   bool graph_on = false;

@@ -10,9 +10,11 @@
 // Vector fields are float4.  pos.w carries lambda between the density/lambda
 // pass and the delta-p pass, and rho afterwards; aux.w carries |omega|.
 //
-// Neighbour lists are stored per 64-particle group, slot-major
-// (list[(s/64)*64*60 + k*64 + s%64]), so lane s of a wave reading its k-th
-// neighbour index reads 64 consecutive words.
+// Neighbour lists are stored per 64-particle group in quads of slots:
+// list[(s/64)*64*60 + (k/4)*256 + (s%64)*4 + k%4], so one 16-byte load gives a
+// lane four consecutive neighbours and a wave's load covers 1 KB contiguously.
+#include <type_traits>
+
 #include "sph_internal.h"
 #include "sph_mlp_dev.h"
 
@@ -32,46 +34,53 @@ __device__ __forceinline__ uint32_t particle_index(uint32_t n_blocks) {
 }
 
 __device__ __forceinline__ const uint32_t *list_ptr(const uint32_t *list, uint32_t s) {
-  return list + (size_t)(s >> 6) * kListStride + (s & 63u);
+  return list + (size_t)(s >> 6) * kListStride + (s & 63u) * 4u;
 }
 
-// Visits the c neighbours of a particle in list order, kBatch at a time: first
-// the batch's list loads, then its dependent gathers, then the arithmetic, so
-// every wave keeps kBatch loads in flight instead of one dependent chain per
-// neighbour.  With PREFETCH the list loads of batch k+1 are issued right after
-// the gathers of batch k, so a batch costs one memory round trip instead of
-// two; that pays for 16-byte gathers (-10% time per pass) and not for the
-// 32-byte (Pair4) ones, whose extra registers cost more occupancy than the
-// hidden latency gains (+30% measured).  Padding slots load the particle itself
-// (`self`, always a valid index) and are skipped by the body.
-constexpr int kBatch = 8;
+// Visits the c neighbours of a particle in list order, NB at a time: first the
+// batch's list quads (NB/4 16-byte loads), then its NB dependent gathers, then
+// the arithmetic.  The gathers are latency-bound, so the depth NB (gathers in
+// flight per wave) sets the speed: 16 was 13-22% faster than 8 in every
+// gather pass on MI355X; the autotuner chooses 8, 16 or 32 per kernel.  With
+// PREFETCH the list quads of batch k+1 are issued right after the gathers of
+// batch k, so a batch costs one memory round trip instead of two; that pays for
+// 16-byte gathers and not for the 32-byte (Pair4) ones.  Slots past c are never
+// used (their gathers load the particle itself, `self`, always a valid index).
+constexpr int kBatch = 8;  // candidate batch of the list build
+constexpr uint32_t kQuadStride = kWave * 4;  // list words between a lane's quads
 
-template <typename T, bool PREFETCH = true, typename Load, typename Body>
+template <int NB>
+__device__ __forceinline__ void load_quads(const uint32_t *L, int k0, int c, uint32_t self,
+                                           uint32_t (&j)[NB]) {
+  const uint4 *Q = reinterpret_cast<const uint4 *>(L);
+  const int q0 = k0 >> 2;
+#pragma unroll
+  for (int u = 0; u < NB / 4; ++u) {
+    uint4 a = make_uint4(self, self, self, self);
+    if (k0 + 4 * u < c) a = Q[(size_t)(q0 + u) * kWave];
+    j[4 * u] = a.x;
+    j[4 * u + 1] = a.y;
+    j[4 * u + 2] = a.z;
+    j[4 * u + 3] = a.w;
+  }
+#pragma unroll
+  for (int t = 0; t < NB; ++t)
+    if (k0 + t >= c) j[t] = self;
+}
+
+template <typename T, int NB, bool PREFETCH = true, typename Load, typename Body>
 __device__ __forceinline__ void for_neighbors(const uint32_t *L, int c, uint32_t self, Load load,
                                               Body body) {
-  uint32_t jn[kBatch];
+  uint32_t jn[NB];
+  load_quads<NB>(L, 0, c, self, jn);
+  for (int k0 = 0; k0 < c; k0 += NB) {
+    if (!PREFETCH && k0 > 0) load_quads<NB>(L, k0, c, self, jn);
+    T q[NB];
 #pragma unroll
-  for (int b = 0; b < kBatch; ++b) jn[b] = (b < c) ? L[(size_t)b * kWave] : self;
-  for (int k0 = 0; k0 < c; k0 += kBatch) {
-    if (!PREFETCH && k0 > 0) {
+    for (int b = 0; b < NB; ++b) q[b] = load(jn[b]);
+    if (PREFETCH) load_quads<NB>(L, k0 + NB, c, self, jn);
 #pragma unroll
-      for (int b = 0; b < kBatch; ++b) {
-        const int k = k0 + b;
-        jn[b] = (k < c) ? L[(size_t)k * kWave] : self;
-      }
-    }
-    T q[kBatch];
-#pragma unroll
-    for (int b = 0; b < kBatch; ++b) q[b] = load(jn[b]);
-    if (PREFETCH) {
-#pragma unroll
-      for (int b = 0; b < kBatch; ++b) {
-        const int k = k0 + kBatch + b;
-        jn[b] = (k < c) ? L[(size_t)k * kWave] : self;
-      }
-    }
-#pragma unroll
-    for (int b = 0; b < kBatch; ++b)
+    for (int b = 0; b < NB; ++b)
       if (k0 + b < c) body(q[b]);
   }
 }
@@ -317,7 +326,7 @@ __global__ __launch_bounds__(kBlock) void k_build_lists(
   uint32_t dropped = 0;
   bool direct = false;
   uint32_t base[3] = {0u, 0u, 0u};
-  uint32_t *out = list + (size_t)(s >> 6) * kListStride + (s & 63u);
+  uint32_t *out = list + (size_t)(s >> 6) * kListStride + (s & 63u) * 4u;
   if (s < n) {
     const float4 p = pos[s];
     const Geo G = geo_of<BATCH>(s, P, B);
@@ -370,7 +379,7 @@ __global__ __launch_bounds__(kBlock) void k_build_lists(
             if (hit) {
               if (c < kMaxNb) {
                 if (direct)
-                  out[(size_t)c * kWave] = j;
+                  out[(size_t)(c >> 2) * kQuadStride + (c & 3)] = j;
                 else
                   sbuf[c * blockDim.x + threadIdx.x] = (uint16_t)(tag | (j - lb));
                 ++c;
@@ -385,15 +394,22 @@ __global__ __launch_bounds__(kBlock) void k_build_lists(
     cnt[s] = c;
   }
   // coalesced write-out: every lane of the wave walks the wave's longest list
+  // one quad at a time; a wave store covers 1 KB of the list contiguously
   int cmax = direct ? 0 : c;
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) cmax = max(cmax, __shfl_xor(cmax, o, 64));
-  for (int k = 0; k < cmax; ++k) {
+  for (int k = 0; k < cmax; k += 4) {
     if (k < c && !direct) {
-      const uint32_t e = sbuf[k * blockDim.x + threadIdx.x];
-      const uint32_t l = e >> 14;
-      const uint32_t b = l == 0 ? base[0] : (l == 1 ? base[1] : base[2]);
-      out[(size_t)k * kWave] = b + (e & (kLayerSpan - 1u));
+      uint32_t v[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const uint32_t e = (k + t < c) ? sbuf[(k + t) * blockDim.x + threadIdx.x] : 0u;
+        const uint32_t l = e >> 14;
+        const uint32_t bb = l == 0 ? base[0] : (l == 1 ? base[1] : base[2]);
+        v[t] = (k + t < c) ? bb + (e & (kLayerSpan - 1u)) : s;
+      }
+      *reinterpret_cast<uint4 *>(out + (size_t)(k >> 2) * kQuadStride) =
+          make_uint4(v[0], v[1], v[2], v[3]);
     }
   }
   add_counter64(stats, STAT_DROPPED, dropped);
@@ -426,7 +442,7 @@ __device__ __forceinline__ bool converged(const uint32_t *stats) {
 // in one gather: rho_i needs only r^2, lambda_i needs rho_i (own) and the
 // spiky gradients.  Writes rho and pos.w = lambda.
 // MODE 0: both, 1: density only (rho), 2: lambda only (reads rho).
-template <int MODE>
+template <int MODE, int NB>
 __global__ __launch_bounds__(kBlock) void k_density_lambda(
     uint32_t n, uint32_t nblocks, float4 *pos, const uint32_t *__restrict__ list,
     const int *__restrict__ cnt, DevParams P, float *rho_io, uint32_t *stats, int err_slot,
@@ -440,7 +456,7 @@ __global__ __launch_bounds__(kBlock) void k_density_lambda(
     const int c = cnt[s];
     const uint32_t *L = list_ptr(list, s);
     float su3 = 0.0f, sgx = 0.0f, sgy = 0.0f, sgz = 0.0f, sc = 0.0f;
-    for_neighbors<float4>(L, c, s, [&](uint32_t j) { return pos[j]; }, [&](const float4 &q) {
+    for_neighbors<float4, NB>(L, c, s, [&](uint32_t j) { return pos[j]; }, [&](const float4 &q) {
       const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
       const float r2 = dx * dx + dy * dy + dz * dz;
       if (MODE != 2) {
@@ -526,10 +542,11 @@ __device__ __forceinline__ void collide(float &x, float &y, float &z, const SdfD
 }
 
 // UpdateDPs (particles.c:668-746): returns sum_j (lambda_i + lambda_j) DelW(r)/r d.
+template <int NB>
 __device__ __forceinline__ float3 dp_sum(const float4 &p, const float4 *pos, const uint32_t *L,
                                          int c, uint32_t self, const DevParams &P) {
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
-  for_neighbors<float4>(L, c, self, [&](uint32_t j) { return pos[j]; }, [&](const float4 &q) {
+  for_neighbors<float4, NB>(L, c, self, [&](uint32_t j) { return pos[j]; }, [&](const float4 &q) {
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     const float r2 = dx * dx + dy * dy + dz * dz;
     const float r2c = fmaxf(r2, P.rmin2);
@@ -545,6 +562,7 @@ __device__ __forceinline__ float3 dp_sum(const float4 &p, const float4 *pos, con
 }
 
 // Staged UpdateDPs: dp = stiffness * m * DelW_norm / rho0 * sum.
+template <int NB>
 __global__ __launch_bounds__(kBlock) void k_dp(uint32_t n, uint32_t nblocks,
                                                const float4 *__restrict__ pos,
                                                const uint32_t *__restrict__ list,
@@ -553,7 +571,7 @@ __global__ __launch_bounds__(kBlock) void k_dp(uint32_t n, uint32_t nblocks,
   const uint32_t s = particle_index(nblocks);
   if (s >= n) return;
   const float4 p = pos[s];
-  const float3 a = dp_sum(p, pos, list_ptr(list, s), cnt[s], s, P);
+  const float3 a = dp_sum<NB>(p, pos, list_ptr(list, s), cnt[s], s, P);
   dp[s] = make_float4(scale * a.x, scale * a.y, scale * a.z, 0.0f);
 }
 
@@ -580,7 +598,7 @@ __global__ __launch_bounds__(kBlock) void k_position_stars(uint32_t n, float4 *_
 // need.  In tolerance mode, once the density pass of this iteration is
 // within tolerance, the particles are copied unchanged instead.  With lam_acc,
 // adds stiff * lambda_i to the particle's accumulated multiplier.
-template <bool COLLIDE, bool BATCH>
+template <bool COLLIDE, bool BATCH, int NB>
 __global__ __launch_bounds__(kBlock) void k_dp_update(
     uint32_t n, uint32_t nblocks, const float4 *__restrict__ pin, float4 *__restrict__ pout,
     const float *__restrict__ rho, const uint32_t *__restrict__ list,
@@ -600,7 +618,7 @@ __global__ __launch_bounds__(kBlock) void k_dp_update(
     return;
   }
   if (s == 0) stats[STAT_ITERS] += 1u;
-  const float3 a = dp_sum(p, pin, list_ptr(list, s), cnt[s], s, P);
+  const float3 a = dp_sum<NB>(p, pin, list_ptr(list, s), cnt[s], s, P);
   float x = p.x + scale * a.x, y = p.y + scale * a.y, z = p.z + scale * a.z;
   apply_boundary(x, y, z, geo_of<BATCH>(s, P, B), P);
   if (COLLIDE) collide(x, y, z, sdf_of<BATCH>(s, o, B));
@@ -646,7 +664,7 @@ __device__ __forceinline__ void sdf_features(float x, float y, float z, const Sd
 // the plain first step plus the predicted remainder.
 // Either way the particle's accumulated multiplier restarts at zero.  Every
 // lane of a wave reaches the network (no early return).
-template <bool LOG, bool BATCH>
+template <bool LOG, bool BATCH, int NB = 8>
 __global__ __launch_bounds__(kBlock) void k_warm_lambda(
     uint32_t n, uint32_t nblocks, float4 *pos, const float4 *__restrict__ xold,
     const uint32_t *__restrict__ list, const int *__restrict__ cnt, DevParams P,
@@ -668,7 +686,7 @@ __global__ __launch_bounds__(kBlock) void k_warm_lambda(
     float su3 = 0.0f, sgx = 0.0f, sgy = 0.0f, sgz = 0.0f, sc = 0.0f;
     float mx = 0.0f, my = 0.0f, mz = 0.0f, dv = 0.0f;
     auto load = [&](uint32_t j) { return Pair4{pos[j], xold[j]}; };
-    for_neighbors<Pair4, false>(list_ptr(list, s), c, s, load, [&](const Pair4 &qq) {
+    for_neighbors<Pair4, NB, false>(list_ptr(list, s), c, s, load, [&](const Pair4 &qq) {
       const float4 &q = qq.a, &qo = qq.b;
       const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
       const float r2 = dx * dx + dy * dy + dz * dz;
@@ -812,7 +830,7 @@ __global__ __launch_bounds__(kBlock) void k_velocity(uint32_t n, const float4 *_
 // ComputeSurfaceTension colour pass (particles.c:94-140), optionally fused
 // with UpdateVelocities (own particle only).  DelWPoly(r)/r = norm (h^2-r^2)^2
 // with r floored at 1e-4 h.
-template <bool WITH_VELOCITY>
+template <bool WITH_VELOCITY, int NB>
 __global__ __launch_bounds__(kBlock) void k_color(uint32_t n, uint32_t nblocks,
                                                   const float4 *__restrict__ pos,
                                                   const float4 *__restrict__ xold,
@@ -827,7 +845,7 @@ __global__ __launch_bounds__(kBlock) void k_color(uint32_t n, uint32_t nblocks,
   const uint32_t *L = list_ptr(list, s);
   const int c = cnt[s];
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
-  for_neighbors<float4>(L, c, s, [&](uint32_t j) { return pos[j]; }, [&](const float4 &q) {
+  for_neighbors<float4, NB>(L, c, s, [&](uint32_t j) { return pos[j]; }, [&](const float4 &q) {
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     const float r2c = fmaxf(dx * dx + dy * dy + dz * dz, P.rmin2);
     const float u = P.h2 - r2c;
@@ -841,6 +859,7 @@ __global__ __launch_bounds__(kBlock) void k_color(uint32_t n, uint32_t nblocks,
 
 // ComputeSurfaceTension force pass (particles.c:142-204): cohesion (Akinci
 // spline C, particles.c:54-65) + curvature, v += dt F / rho_i in place.
+template <int NB>
 __global__ __launch_bounds__(kBlock) void k_surface(uint32_t n, uint32_t nblocks,
                                                     const float4 *__restrict__ pos,
                                                     const float4 *__restrict__ color,
@@ -856,7 +875,7 @@ __global__ __launch_bounds__(kBlock) void k_surface(uint32_t n, uint32_t nblocks
   const float hh = 0.5f * P.h;
   float fx = 0.0f, fy = 0.0f, fz = 0.0f;
   auto load = [&](uint32_t j) { return Pair4{pos[j], color[j]}; };
-  for_neighbors<Pair4, false>(L, c, s, load, [&](const Pair4 &qc) {
+  for_neighbors<Pair4, NB, false>(L, c, s, load, [&](const Pair4 &qc) {
     const float4 &q = qc.a, &cq = qc.b;
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     const float r2c = fmaxf(dx * dx + dy * dy + dz * dz, P.rmin2);
@@ -884,6 +903,7 @@ __global__ __launch_bounds__(kBlock) void k_surface(uint32_t n, uint32_t nblocks
 
 // ApplyViscosity (particles.c:383-459), reading v_old and writing v_new.  The
 // distance keeps the reference's typo r^2 = dx^2 + dy^2 dz^2 (particles.c:439).
+template <int NB>
 __global__ __launch_bounds__(kBlock) void k_xsph(uint32_t n, uint32_t nblocks,
                                                  const float4 *__restrict__ pos,
                                                  const float4 *__restrict__ vin,
@@ -898,7 +918,7 @@ __global__ __launch_bounds__(kBlock) void k_xsph(uint32_t n, uint32_t nblocks,
   const int c = cnt[s];
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
   auto load = [&](uint32_t j) { return Pair4{pos[j], vin[j]}; };
-  for_neighbors<Pair4, false>(L, c, s, load, [&](const Pair4 &qv) {
+  for_neighbors<Pair4, NB, false>(L, c, s, load, [&](const Pair4 &qv) {
     const float4 &q = qv.a, &vq = qv.b;
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     const float u = P.h2 - (dx * dx + dy * dy * dz * dz);
@@ -913,6 +933,7 @@ __global__ __launch_bounds__(kBlock) void k_xsph(uint32_t n, uint32_t nblocks,
 
 // ComputeVorticity (particles.c:207-283); aux = (omega, |omega|), and
 // pw = (x*, |omega|) so that the confinement gather is one float4 per neighbour.
+template <int NB>
 __global__ __launch_bounds__(kBlock) void k_vorticity(uint32_t n, uint32_t nblocks,
                                                       const float4 *__restrict__ pos,
                                                       const float4 *__restrict__ vel,
@@ -928,7 +949,7 @@ __global__ __launch_bounds__(kBlock) void k_vorticity(uint32_t n, uint32_t nbloc
   const int c = cnt[s];
   float ox = 0.0f, oy = 0.0f, oz = 0.0f;
   auto load = [&](uint32_t j) { return Pair4{pos[j], vel[j]}; };
-  for_neighbors<Pair4, false>(L, c, s, load, [&](const Pair4 &qv) {
+  for_neighbors<Pair4, NB, false>(L, c, s, load, [&](const Pair4 &qv) {
     const float4 &q = qv.a, &vq = qv.b;
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     const float r2c = fmaxf(dx * dx + dy * dy + dz * dz, P.rmin2);
@@ -955,7 +976,7 @@ __global__ __launch_bounds__(kBlock) void k_vorticity(uint32_t n, uint32_t nbloc
 // N = eta / max(|eta|, 1e-4 h), v += dt eps (N x omega_i) / m.  When COMMIT
 // is set the result is also UpdatePositions (particles.c:566-586): x* and v
 // are scattered back to particle-id order.
-template <bool COMMIT>
+template <bool COMMIT, int NB>
 __global__ __launch_bounds__(kBlock) void k_confine(
     uint32_t n, uint32_t nblocks, const float4 *__restrict__ pw,
     const float4 *__restrict__ omega, const uint32_t *__restrict__ list,
@@ -969,7 +990,7 @@ __global__ __launch_bounds__(kBlock) void k_confine(
   const int c = cnt[s];
   float ex = 0.0f, ey = 0.0f, ez = 0.0f;
   // only x_j and |omega_j| of the neighbour are needed
-  for_neighbors<float4>(L, c, s, [&](uint32_t j) { return pw[j]; }, [&](const float4 &q) {
+  for_neighbors<float4, NB>(L, c, s, [&](uint32_t j) { return pw[j]; }, [&](const float4 &q) {
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     const float r2c = fmaxf(dx * dx + dy * dy + dz * dz, P.rmin2);
     const float ir = fast_rsq(r2c);
@@ -1110,6 +1131,22 @@ __global__ void k_zero_f64(double *p) { *p = 0.0; }
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
+// Gathers in flight per wave of a gather kernel: the autotuner's choice or 16.
+static inline int tuned_depth(const sph_ctx *c, int kernel) {
+  const int d = c->tune.depth[kernel];
+  return d ? d : 16;
+}
+
+template <typename F>
+static inline void with_depth(int depth, F &&f) {
+  if (depth == 32)
+    f(std::integral_constant<int, 32>{});
+  else if (depth == 8)
+    f(std::integral_constant<int, 8>{});
+  else
+    f(std::integral_constant<int, 16>{});
+}
+
 // Threads per block of a gather kernel: the autotuner's choice or kBlock.
 static inline uint32_t tuned_block(const sph_ctx *c, int kernel) {
   const uint32_t b = c->tune.block[kernel];
@@ -1234,24 +1271,36 @@ int launch_density(sph_ctx *c, int mode, int err_slot) {
   const uint32_t nb = blocks_for(n, bs);
   hipStream_t s = c->stream;
   if (mode == 0)
-    LAUNCH_TIMED(c, K_DENSITY_LAMBDA, 0,
-                 k_density_lambda<0><<<nb, bs, 0, s>>>(n, nb, c->pos, c->list, c->list_cnt,
-                                                           c->dev, c->rho, c->stats, err_slot,
-                                                           c->P.density_tolerance > 0, 0));
+    with_depth(tuned_depth(c, K_DENSITY_LAMBDA), [&](auto D_) {
+      constexpr int NBv = decltype(D_)::value;
+      LAUNCH_TIMED(c, K_DENSITY_LAMBDA, 0,
+                   k_density_lambda<0, NBv><<<nb, bs, 0, s>>>(n, nb, c->pos, c->list, c->list_cnt,
+                                                             c->dev, c->rho, c->stats, err_slot,
+                                                             c->P.density_tolerance > 0, 0));
+    });
   else if (mode == 1)
-    LAUNCH_TIMED(c, K_DENSITY, 0,
-                 k_density_lambda<1><<<nb, bs, 0, s>>>(n, nb, c->pos, c->list, c->list_cnt,
-                                                           c->dev, c->rho, c->stats, err_slot, 0,
-                                                           0));
+    with_depth(tuned_depth(c, K_DENSITY), [&](auto D_) {
+      constexpr int NBv = decltype(D_)::value;
+      LAUNCH_TIMED(c, K_DENSITY, 0,
+                   k_density_lambda<1, NBv><<<nb, bs, 0, s>>>(n, nb, c->pos, c->list, c->list_cnt,
+                                                             c->dev, c->rho, c->stats, err_slot, 0,
+                                                             0));
+    });
   else if (mode == 3)  // final density of a capped tolerance loop: rho also into w
-    LAUNCH_TIMED(c, K_DENSITY, 0,
-                 k_density_lambda<1><<<nb, bs, 0, s>>>(n, nb, c->pos, c->list, c->list_cnt,
-                                                           c->dev, c->rho, c->stats, err_slot, 1,
-                                                           1));
+    with_depth(tuned_depth(c, K_DENSITY), [&](auto D_) {
+      constexpr int NBv = decltype(D_)::value;
+      LAUNCH_TIMED(c, K_DENSITY, 0,
+                   k_density_lambda<1, NBv><<<nb, bs, 0, s>>>(n, nb, c->pos, c->list, c->list_cnt,
+                                                             c->dev, c->rho, c->stats, err_slot, 1,
+                                                             1));
+    });
   else
-    LAUNCH_TIMED(c, K_LAMBDA, 0,
-                 k_density_lambda<2><<<nb, bs, 0, s>>>(n, nb, c->pos, c->list, c->list_cnt,
-                                                           c->dev, c->rho, c->stats, -1, 0, 0));
+    with_depth(tuned_depth(c, K_LAMBDA), [&](auto D_) {
+      constexpr int NBv = decltype(D_)::value;
+      LAUNCH_TIMED(c, K_LAMBDA, 0,
+                   k_density_lambda<2, NBv><<<nb, bs, 0, s>>>(n, nb, c->pos, c->list, c->list_cnt,
+                                                             c->dev, c->rho, c->stats, -1, 0, 0));
+    });
   SPH_HIP(hipGetLastError());
   return SPH_OK;
 }
@@ -1261,9 +1310,12 @@ int launch_dp(sph_ctx *c, int substep) {
   if (!n) return SPH_OK;
   const uint32_t bs = tuned_block(c, K_DP);
   const uint32_t nb = blocks_for(n, bs);
-  LAUNCH_TIMED(c, K_DP, 0,
-               k_dp<<<nb, bs, 0, c->stream>>>(n, nb, c->pos, c->list, c->list_cnt, c->dev,
+  with_depth(tuned_depth(c, K_DP), [&](auto D_) {
+    constexpr int NBv = decltype(D_)::value;
+    LAUNCH_TIMED(c, K_DP, 0,
+               k_dp<NBv><<<nb, bs, 0, c->stream>>>(n, nb, c->pos, c->list, c->list_cnt, c->dev,
                                                   stiffness_scale(c, substep), c->dpbuf));
+  });
   SPH_HIP(hipGetLastError());
   return SPH_OK;
 }
@@ -1299,11 +1351,14 @@ int launch_dp_update(sph_ctx *c, int substep, int err_slot, bool accumulate) {
   const BatchDev B = c->batch();
   const bool obstacle = c->nscenes ? c->batch_sdf != nullptr : c->sdf.data != nullptr;
   const bool split = obstacle && c->tune.split_collide;
-#define DP_UPDATE(COL, BAT)                                                              \
-  LAUNCH_TIMED(c, K_DP_UPDATE, 0,                                                        \
-               (k_dp_update<COL, BAT><<<nb, bs, 0, c->stream>>>(                     \
-                   n, nb, c->pos, out, c->rho, c->list, c->list_cnt, c->dev, scale, c->sdf, \
-                   c->stats, err_slot, tol, acc, stiff, B)))
+#define DP_UPDATE(COL, BAT)                                                                 \
+  with_depth(tuned_depth(c, K_DP_UPDATE), [&](auto D_) {                                    \
+    constexpr int NBv = decltype(D_)::value;                                                \
+    LAUNCH_TIMED(c, K_DP_UPDATE, 0,                                                         \
+                 (k_dp_update<COL, BAT, NBv><<<nb, bs, 0, c->stream>>>(                      \
+                     n, nb, c->pos, out, c->rho, c->list, c->list_cnt, c->dev, scale, c->sdf, \
+                     c->stats, err_slot, tol, acc, stiff, B)));                             \
+  })
   if (c->nscenes) {
     if (split) DP_UPDATE(false, true); else DP_UPDATE(true, true);
   } else {
@@ -1411,15 +1466,21 @@ int launch_color(sph_ctx *c, bool with_velocity) {
   const uint32_t bs = tuned_block(c, K_COLOR);
   const uint32_t nb = blocks_for(n, bs);
   if (with_velocity)
-    LAUNCH_TIMED(c, K_COLOR, 0,
-                 k_color<true><<<nb, bs, 0, c->stream>>>(n, nb, c->pos, c->xold, c->list,
-                                                             c->list_cnt, c->dev, c->vel,
-                                                             c->aux));
+    with_depth(tuned_depth(c, K_COLOR), [&](auto D_) {
+      constexpr int NBv = decltype(D_)::value;
+      LAUNCH_TIMED(c, K_COLOR, 0,
+                   k_color<true, NBv><<<nb, bs, 0, c->stream>>>(n, nb, c->pos, c->xold, c->list,
+                                                               c->list_cnt, c->dev, c->vel,
+                                                               c->aux));
+    });
   else
-    LAUNCH_TIMED(c, K_COLOR, 0,
-                 k_color<false><<<nb, bs, 0, c->stream>>>(n, nb, c->pos, c->xold, c->list,
-                                                              c->list_cnt, c->dev, c->vel,
-                                                              c->aux));
+    with_depth(tuned_depth(c, K_COLOR), [&](auto D_) {
+      constexpr int NBv = decltype(D_)::value;
+      LAUNCH_TIMED(c, K_COLOR, 0,
+                   k_color<false, NBv><<<nb, bs, 0, c->stream>>>(n, nb, c->pos, c->xold, c->list,
+                                                                c->list_cnt, c->dev, c->vel,
+                                                                c->aux));
+    });
   SPH_HIP(hipGetLastError());
   return SPH_OK;
 }
@@ -1429,9 +1490,12 @@ int launch_surface(sph_ctx *c) {
   if (!n) return SPH_OK;
   const uint32_t bs = tuned_block(c, K_SURFACE);
   const uint32_t nb = blocks_for(n, bs);
-  LAUNCH_TIMED(c, K_SURFACE, 0,
-               k_surface<<<nb, bs, 0, c->stream>>>(n, nb, c->pos, c->aux, c->list,
+  with_depth(tuned_depth(c, K_SURFACE), [&](auto D_) {
+    constexpr int NBv = decltype(D_)::value;
+    LAUNCH_TIMED(c, K_SURFACE, 0,
+               k_surface<NBv><<<nb, bs, 0, c->stream>>>(n, nb, c->pos, c->aux, c->list,
                                                        c->list_cnt, c->dev, c->vel));
+  });
   SPH_HIP(hipGetLastError());
   return SPH_OK;
 }
@@ -1442,9 +1506,12 @@ int launch_xsph(sph_ctx *c) {
   if (n) {
     const uint32_t bs = tuned_block(c, K_XSPH);
   const uint32_t nb = blocks_for(n, bs);
-    LAUNCH_TIMED(c, K_XSPH, 0,
-                 k_xsph<<<nb, bs, 0, c->stream>>>(n, nb, c->pos, c->vel, c->list,
-                                                      c->list_cnt, c->dev, out));
+    with_depth(tuned_depth(c, K_XSPH), [&](auto D_) {
+      constexpr int NBv = decltype(D_)::value;
+      LAUNCH_TIMED(c, K_XSPH, 0,
+                   k_xsph<NBv><<<nb, bs, 0, c->stream>>>(n, nb, c->pos, c->vel, c->list,
+                                                        c->list_cnt, c->dev, out));
+    });
     SPH_HIP(hipGetLastError());
   }
   c->vel = out;
@@ -1456,9 +1523,12 @@ int launch_vorticity(sph_ctx *c) {
   if (!n) return SPH_OK;
   const uint32_t bs = tuned_block(c, K_VORTICITY);
   const uint32_t nb = blocks_for(n, bs);
-  LAUNCH_TIMED(c, K_VORTICITY, 0,
-               k_vorticity<<<nb, bs, 0, c->stream>>>(n, nb, c->pos, c->vel, c->list,
+  with_depth(tuned_depth(c, K_VORTICITY), [&](auto D_) {
+    constexpr int NBv = decltype(D_)::value;
+    LAUNCH_TIMED(c, K_VORTICITY, 0,
+               k_vorticity<NBv><<<nb, bs, 0, c->stream>>>(n, nb, c->pos, c->vel, c->list,
                                                          c->list_cnt, c->dev, c->aux, c->pw));
+  });
   SPH_HIP(hipGetLastError());
   return SPH_OK;
 }
@@ -1470,15 +1540,21 @@ int launch_confine(sph_ctx *c, bool commit) {
     const uint32_t bs = tuned_block(c, K_CONFINE);
   const uint32_t nb = blocks_for(n, bs);
     if (commit)
-      LAUNCH_TIMED(c, K_CONFINE, 0,
-                   k_confine<true><<<nb, bs, 0, c->stream>>>(
-                       n, nb, c->pw, c->aux, c->list, c->list_cnt, c->dev, c->vel, out,
-                       c->id_s, c->x_can, c->v_can));
+      with_depth(tuned_depth(c, K_CONFINE), [&](auto D_) {
+        constexpr int NBv = decltype(D_)::value;
+        LAUNCH_TIMED(c, K_CONFINE, 0,
+                       k_confine<true, NBv><<<nb, bs, 0, c->stream>>>(
+                           n, nb, c->pw, c->aux, c->list, c->list_cnt, c->dev, c->vel, out,
+                           c->id_s, c->x_can, c->v_can));
+      });
     else
-      LAUNCH_TIMED(c, K_CONFINE, 0,
-                   k_confine<false><<<nb, bs, 0, c->stream>>>(
-                       n, nb, c->pw, c->aux, c->list, c->list_cnt, c->dev, c->vel, out,
-                       c->id_s, c->x_can, c->v_can));
+      with_depth(tuned_depth(c, K_CONFINE), [&](auto D_) {
+        constexpr int NBv = decltype(D_)::value;
+        LAUNCH_TIMED(c, K_CONFINE, 0,
+                       k_confine<false, NBv><<<nb, bs, 0, c->stream>>>(
+                           n, nb, c->pw, c->aux, c->list, c->list_cnt, c->dev, c->vel, out,
+                           c->id_s, c->x_can, c->v_can));
+      });
     SPH_HIP(hipGetLastError());
   }
   if (!commit) c->vel = out;

@@ -93,6 +93,7 @@ def test_autotune_keeps_state_and_results():
     p1, v1 = tuned.download()
     assert np.array_equal(p0, p1) and np.array_equal(v0, v1)
     assert all(b in (64, 128, 256) for b in t["block"].values())
+    assert all(d in (8, 16, 32) for d in t["depth"].values())
     assert t["ms_per_substep_tuned"] > 0
     for step in range(3):
         ref.step(1)
