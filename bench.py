@@ -151,6 +151,7 @@ def compact_tuning(t):
     return {"ms_per_substep_default": t["ms_per_substep_default"],
             "ms_per_substep_tuned": t["ms_per_substep_tuned"],
             "split_collide": t["split_collide"],
+            "list_format": "wide" if t["wide_lists"] else "compact",
             "threads_per_block": {k: t["block"][k] for k in GATHER_KERNELS},
             "gathers_in_flight": {k: t["depth"][k] for k in GATHER_KERNELS if k != "build_lists"}}
 

@@ -298,18 +298,26 @@ int sph_profile_kernel_count(void);
  * Times `substeps` substeps for each candidate launch configuration from the
  * context's current state (restored afterwards): 64/128/256 threads per block
  * and 8/16/32 neighbour gathers in flight per wavefront for every
- * neighbour-gather kernel, and, with an obstacle, the push fused into the
- * position update or as a separate pass.  Keeps the fastest choice per
+ * neighbour-gather kernel, with an obstacle the push fused into the
+ * position update or as a separate pass, and compact (16-bit offsets) versus
+ * wide (32-bit indices) neighbour lists.  Keeps the fastest choice per
  * kernel.  Results do not depend on the choice. */
 #define SPH_TUNE_KERNELS 32
 typedef struct sph_tuning {
   int split_collide;              /* obstacle push as a separate pass */
+  int wide_lists;                 /* 32-bit list entries for every wavefront */
   int block[SPH_TUNE_KERNELS];    /* threads per block, by sph_profile_kernel_name index */
   int depth[SPH_TUNE_KERNELS];    /* gathers in flight per wavefront, same index */
   double ms_per_substep_default;  /* event-timed kernel sum before tuning */
   double ms_per_substep_tuned;    /* and with the chosen configuration */
 } sph_tuning;
 int sph_autotune(sph_ctx *ctx, int substeps, sph_tuning *result);
+/* The context's current launch choices (timings zero), and applying a saved
+ * set: block 0 / depth 0 keep the default (256 threads, 16 gathers); other
+ * values must be 64, 128 or 256 and 8, 16 or 32.  Entries of kernels that
+ * are not tuned are ignored. */
+int sph_get_tuning(sph_ctx *ctx, sph_tuning *out);
+int sph_set_tuning(sph_ctx *ctx, const sph_tuning *tuning);
 
 /* --- learned lambda warm start ------------------------------------------
  * No counterpart in the reference: a 12 -> 64 -> 64 -> 1 SiLU network that

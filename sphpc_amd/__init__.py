@@ -59,8 +59,9 @@ class Scene(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("split_collide", C.c_int), ("block", C.c_int * 32), ("depth", C.c_int * 32),
-                ("ms_per_substep_default", C.c_double), ("ms_per_substep_tuned", C.c_double)]
+    _fields_ = [("split_collide", C.c_int), ("wide_lists", C.c_int), ("block", C.c_int * 32),
+                ("depth", C.c_int * 32), ("ms_per_substep_default", C.c_double),
+                ("ms_per_substep_tuned", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -149,6 +150,8 @@ def lib():
     L.sph_set_warm_start.argtypes = [vp, vp]
     L.sph_set_graph.argtypes = [vp, ip]
     L.sph_autotune.argtypes = [vp, ip, C.POINTER(Tuning)]
+    L.sph_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
+    L.sph_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
     L.sph_create_batch.argtypes = [C.POINTER(Params), ip, C.POINTER(Scene), C.POINTER(vp)]
     L.sph_batch_scene_count.argtypes = [vp, C.POINTER(ip)]
     L.sph_batch_scene_errors.argtypes = [vp, fp]
@@ -300,16 +303,42 @@ class Context:
         check(lib().sph_compute_energy(self._h, C.byref(e)))
         return e.value
 
-    def autotune(self, substeps=3):
-        """Launch autotuner; returns {kernel name: threads per block}, split, timings."""
-        t = Tuning()
-        check(lib().sph_autotune(self._h, int(substeps), C.byref(t)))
+    @staticmethod
+    def _tuning_dict(t):
         names = kernel_names()
-        blocks = {names[k]: t.block[k] for k in range(len(names))}
-        depths = {names[k]: t.depth[k] for k in range(len(names))}
-        return dict(block=blocks, depth=depths, split_collide=bool(t.split_collide),
+        return dict(block={names[k]: t.block[k] for k in range(len(names))},
+                    depth={names[k]: t.depth[k] for k in range(len(names))},
+                    split_collide=bool(t.split_collide), wide_lists=bool(t.wide_lists),
                     ms_per_substep_default=t.ms_per_substep_default,
                     ms_per_substep_tuned=t.ms_per_substep_tuned)
+
+    def autotune(self, substeps=3):
+        """Launch autotuner; returns {kernel name: threads per block}, gathers in
+        flight, obstacle-push split, list format and timings."""
+        t = Tuning()
+        check(lib().sph_autotune(self._h, int(substeps), C.byref(t)))
+        return self._tuning_dict(t)
+
+    def tuning(self):
+        """Current launch choices, as autotune returns them (timings zero)."""
+        t = Tuning()
+        check(lib().sph_get_tuning(self._h, C.byref(t)))
+        return self._tuning_dict(t)
+
+    def set_tuning(self, block=None, depth=None, split_collide=None, wide_lists=None):
+        """Applies launch choices; dicts map kernel names to values, unnamed
+        kernels keep their current choice."""
+        t = Tuning()
+        check(lib().sph_get_tuning(self._h, C.byref(t)))
+        names = kernel_names()
+        for src, dst in ((block, t.block), (depth, t.depth)):
+            for name, v in (src or {}).items():
+                dst[names.index(name)] = int(v)
+        if split_collide is not None:
+            t.split_collide = int(bool(split_collide))
+        if wide_lists is not None:
+            t.wide_lists = int(bool(wide_lists))
+        check(lib().sph_set_tuning(self._h, C.byref(t)))
 
     def residuals(self):
         """Max density error of each density pass of the last fused substep."""

@@ -172,7 +172,7 @@ static void free_all(sph_ctx *c) {
   graph_reset(c);
   void *bufs[] = {c->x_can, c->v_can, c->xs_can, c->keys, c->keys_alt, c->ids, c->ids_alt,
                   c->hist, c->cell_count, c->cell_start, c->pos_a, c->pos_b, c->xold,
-                  c->vel_a, c->vel_b, c->aux, c->dpbuf, c->rho, c->list, c->list_cnt,
+                  c->vel_a, c->vel_b, c->aux, c->dpbuf, c->rho, c->list, c->list_head,
                   c->stats, c->pair_partial, c->red, c->sdf_buf, c->scan.tile_sums,
                   c->pw, c->lam_acc, c->lam_prev, c->feat, c->scenes_dev, c->scene_of,
                   c->batch_sdf, c->scene_err,
@@ -273,12 +273,13 @@ static int ctx_allocate(sph_ctx *c, int max_particles, sph_ctx **out) {
       {(void **)&c->keys_alt, u4},    {(void **)&c->ids, u4},
       {(void **)&c->ids_alt, u4},     {(void **)&c->cell_count, sizeof(uint32_t) * (c->cells + 1ull)},
       {(void **)&c->cell_start, sizeof(uint32_t) * (c->cells + 1ull)},
-      {(void **)&c->pos_a, f4},       {(void **)&c->pos_b, f4},
+      {(void **)&c->pos_a, f4 + sizeof(float4) * kPosPad},
+      {(void **)&c->pos_b, f4 + sizeof(float4) * kPosPad},
       {(void **)&c->xold, f4},        {(void **)&c->vel_a, f4},
       {(void **)&c->vel_b, f4},       {(void **)&c->aux, f4},
       {(void **)&c->dpbuf, f4},       {(void **)&c->rho, sizeof(float) * cap},
       {(void **)&c->list, sizeof(uint32_t) * groups * kListStride},
-      {(void **)&c->list_cnt, sizeof(int) * cap},
+      {(void **)&c->list_head, sizeof(uint4) * cap},
       {(void **)&c->pw, f4},
       {(void **)&c->stats, sizeof(uint32_t) * STAT_WORDS},
       {(void **)&c->pair_partial, sizeof(uint32_t) * blocks_for((uint32_t)cap, kWave)},
@@ -290,7 +291,9 @@ static int ctx_allocate(sph_ctx *c, int max_particles, sph_ctx **out) {
   }
   e = hipMemsetAsync(c->cell_count, 0, sizeof(uint32_t) * (c->cells + 1ull), c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(c->stats, 0, sizeof(uint32_t) * STAT_WORDS, c->stream);
-  if (e == hipSuccess) e = hipMemsetAsync(c->list_cnt, 0, sizeof(int) * cap, c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(c->list_head, 0, sizeof(uint4) * cap, c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(c->pos_a, 0, f4 + sizeof(float4) * kPosPad, c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(c->pos_b, 0, f4 + sizeof(float4) * kPosPad, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return fail(hip_fail(e, "context initialisation"));
   c->pos = c->pos_a;
@@ -1002,17 +1005,28 @@ extern "C" int sph_download_neighbors(sph_ctx *c, uint32_t *out_ids, int *counts
   if (!out_ids || !counts) return arg_error("NULL argument");
   const int n = c->n;
   const size_t groups = ((size_t)n + kWave - 1) / kWave;
-  std::vector<uint32_t> ids, list;
-  std::vector<int> cnt;
+  std::vector<uint32_t> ids, list, head;
   if ((rc = fetch(c, c->id_s, (size_t)n, &ids))) return rc;
-  if ((rc = fetch(c, (const int *)c->list_cnt, (size_t)n, &cnt))) return rc;
+  if ((rc = fetch(c, (const uint32_t *)c->list_head, (size_t)n * 4, &head))) return rc;
   if ((rc = fetch(c, (const uint32_t *)c->list, groups * kListStride, &list))) return rc;
   for (int s = 0; s < n; ++s) {
     const uint32_t id = ids[s];
-    counts[id] = cnt[s];
+    const uint32_t *h = &head[(size_t)s * 4];
+    const int cnt = (int)(h[3] & kHeadCount);
+    const int e0 = (int)((h[3] >> 8) & kHeadCount), e1 = (int)((h[3] >> 16) & kHeadCount);
+    counts[id] = cnt;
     const size_t base = (size_t)(s >> 6) * kListStride + (size_t)(s & 63) * 4;
-    for (int k = 0; k < cnt[s]; ++k)
-      out_ids[(size_t)id * kMaxNb + k] = ids[list[base + (size_t)(k >> 2) * kWave * 4 + (k & 3)]];
+    for (int k = 0; k < cnt; ++k) {
+      uint32_t j;
+      if (h[3] & kHeadWide) {
+        j = list[base + (size_t)(k >> 2) * kWave * 4 + (k & 3)];
+      } else {
+        const uint32_t w = list[base + (size_t)(k >> 3) * kWave * 4 + ((k & 7) >> 1)];
+        j = (k < e0 ? h[0] : (k < e1 ? h[1] : h[2])) + ((k & 1) ? w >> 16 : w & 0xFFFFu);
+      }
+      if (j >= (uint32_t)n) return state_error("neighbour list entry out of range");
+      out_ids[(size_t)id * kMaxNb + k] = ids[j];
+    }
   }
   return SPH_OK;
 }
@@ -1216,6 +1230,17 @@ extern "C" int sph_autotune(sph_ctx *c, int substeps, sph_tuning *out) {
     after = obstacle ? std::min(t_split[0], t_split[1]) : t_split[0];
     c->tune.split_collide = split_best;
   }
+  if (!rc && e == hipSuccess) {
+    // list format: the other one than the trials above used
+    const bool wide0 = c->tune.wide_lists;
+    c->tune.wide_lists = !wide0;
+    graph_reset(c);
+    rc = trial(cur, &total);
+    if (!rc && total < after)
+      after = total;
+    else
+      c->tune.wide_lists = wide0;
+  }
   if (rc || e != hipSuccess) c->tune = tune0;
   const hipError_t er = copy_state(false);
   release();
@@ -1228,15 +1253,42 @@ extern "C" int sph_autotune(sph_ctx *c, int substeps, sph_tuning *out) {
   if (e != hipSuccess) return hip_fail(e, "autotune snapshot");
   if (er != hipSuccess) return hip_fail(er, "autotune restore");
   if (out) {
-    memset(out, 0, sizeof(*out));
-    out->split_collide = split_best;
-    for (int k = 0; k < K_COUNT && k < SPH_TUNE_KERNELS; ++k) {
-      out->block[k] = c->tune.block[k] ? c->tune.block[k] : kBlock;
-      out->depth[k] = c->tune.depth[k] ? c->tune.depth[k] : 16;
-    }
+    sph_get_tuning(c, out);
     out->ms_per_substep_default = before;
     out->ms_per_substep_tuned = after;
   }
+  return SPH_OK;
+}
+
+extern "C" int sph_get_tuning(sph_ctx *c, sph_tuning *out) {
+  if (!c || !out) return arg_error("NULL argument");
+  memset(out, 0, sizeof(*out));
+  out->split_collide = c->tune.split_collide;
+  out->wide_lists = c->tune.wide_lists;
+  for (int k = 0; k < K_COUNT && k < SPH_TUNE_KERNELS; ++k) {
+    out->block[k] = c->tune.block[k] ? c->tune.block[k] : kBlock;
+    out->depth[k] = c->tune.depth[k] ? c->tune.depth[k] : 16;
+  }
+  return SPH_OK;
+}
+
+extern "C" int sph_set_tuning(sph_ctx *c, const sph_tuning *t) {
+  if (!c || !t) return arg_error("NULL argument");
+  auto tune = c->tune;
+  for (int k : kTunable) {
+    const int b = t->block[k], d = t->depth[k];
+    if (b != 0 && b != 64 && b != 128 && b != 256)
+      return arg_error("threads per block must be 0, 64, 128 or 256");
+    if (d != 0 && d != 8 && d != 16 && d != 32)
+      return arg_error("gathers in flight must be 0, 8, 16 or 32");
+    tune.block[k] = (uint16_t)b;
+    tune.depth[k] = (uint8_t)d;
+  }
+  tune.split_collide = t->split_collide != 0;
+  tune.wide_lists = t->wide_lists != 0;
+  c->tune = tune;
+  // lists already built keep their format; the next search uses the new one
+  graph_reset(c);
   return SPH_OK;
 }
 

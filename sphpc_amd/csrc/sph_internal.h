@@ -14,7 +14,19 @@ namespace sph {
 constexpr int kMaxNb = SPH_MAX_NEIGHBORS;
 constexpr int kBlock = 256;          // threads per block for elementwise kernels
 constexpr int kWave = 64;            // CDNA wavefront
-constexpr int kListStride = kWave * kMaxNb;  // list entries per 64-particle group
+constexpr int kListStride = kWave * kMaxNb;  // list words per 64-particle group
+// Neighbour-list head per particle: the first candidate index of each of the
+// three dz layers and kHeadCount | layer-0 count << 8 | layers-0..1 count << 16
+// | kHeadWide.  A compact list holds 16-bit offsets from the layer's first
+// index; a wide one (whole wavefronts, when a layer spans more than
+// kCompactSpan indices or the tuning asks for it) holds 32-bit indices.
+constexpr uint32_t kHeadCount = 0xFFu;
+constexpr uint32_t kHeadWide = 1u << 31;
+constexpr uint32_t kCompactSpan = 1u << 16;
+// The sorted position arrays hold kPosPad readable entries past the capacity:
+// the list build loads candidates 8 at a time and may read up to 7 entries
+// past the end of a range (their tests are discarded).
+constexpr size_t kPosPad = 8;
 
 
 // Device-side constants of one run, all FP32 except the exact-predicate h^2.
@@ -159,7 +171,7 @@ struct sph_ctx {
   float4 *dpbuf = nullptr;  // staged delta p
   float *rho = nullptr;
   uint32_t *list = nullptr;     // sorted neighbour indices, quads of slots per 64 particles
-  int *list_cnt = nullptr;
+  uint4 *list_head = nullptr;   // per sorted particle: layer bases and counts
   uint32_t *stats = nullptr;  // sph::STAT_WORDS words
   uint32_t *pair_partial = nullptr;  // per-block list-length sums of the last search
   uint32_t pair_blocks = 0;
@@ -185,6 +197,7 @@ struct sph_ctx {
     bool split_collide = true;  // obstacle push as its own pass after k_dp_update
     uint16_t block[sph::K_COUNT] = {};  // threads per block of gather kernels (0: 256)
     uint8_t depth[sph::K_COUNT] = {};   // gathers in flight per wave: 8, 16, 32 (0: 16)
+    bool wide_lists = false;  // 32-bit list entries for every wave
   } tune;
   // whole-substep HIP graph (sph_set_graph)
   bool graph_on = false;

@@ -10,9 +10,14 @@
 // Vector fields are float4.  pos.w carries lambda between the density/lambda
 // pass and the delta-p pass, and rho afterwards; aux.w carries |omega|.
 //
-// Neighbour lists are stored per 64-particle group in quads of slots:
-// list[(s/64)*64*60 + (k/4)*256 + (s%64)*4 + k%4], so one 16-byte load gives a
-// lane four consecutive neighbours and a wave's load covers 1 KB contiguously.
+// Neighbour lists are stored per 64-particle group in 16-byte quads: lane
+// s%64's quad q is at word (s/64)*64*60 + q*256 + (s%64)*4, so a wave's load
+// of one quad per lane covers 1 KB contiguously.  In (dz, dy, dx, id) order a
+// particle's neighbours have strictly increasing sorted indices, and those of
+// one dz layer lie within a few grid rows of the layer's first candidate, so
+// a compact list stores 16-bit offsets from the three layer bases held in the
+// particle's list head (8 neighbours per quad); a wide list stores the 32-bit
+// indices (4 per quad).
 #include <type_traits>
 
 #include "sph_internal.h"
@@ -37,27 +42,50 @@ __device__ __forceinline__ const uint32_t *list_ptr(const uint32_t *list, uint32
   return list + (size_t)(s >> 6) * kListStride + (s & 63u) * 4u;
 }
 
+// One particle's list: its first quad, its head and the wave's format.
+struct NbList {
+  const uint32_t *L;
+  uint4 h;      // layer bases (x, y, z) and counts (w)
+  int c;        // neighbours
+  int e0, e1;   // neighbours in layer 0, in layers 0 and 1
+  bool wide;    // uniform over the wavefront
+};
+
+__device__ __forceinline__ NbList nb_list(const uint32_t *list, const uint4 *head, uint32_t s) {
+  NbList r;
+  r.L = list_ptr(list, s);
+  r.h = head[s];
+  r.c = (int)(r.h.w & kHeadCount);
+  r.e0 = (int)((r.h.w >> 8) & kHeadCount);
+  r.e1 = (int)((r.h.w >> 16) & kHeadCount);
+  // every lane of a wavefront holds the same format bit
+  r.wide = (__builtin_amdgcn_readfirstlane(r.h.w) & kHeadWide) != 0u;
+  return r;
+}
+
 // Visits the c neighbours of a particle in list order, NB at a time: first the
-// batch's list quads (NB/4 16-byte loads), then its NB dependent gathers, then
-// the arithmetic.  The gathers are latency-bound, so the depth NB (gathers in
-// flight per wave) sets the speed: 16 was 13-22% faster than 8 in every
-// gather pass on MI355X; the autotuner chooses 8, 16 or 32 per kernel.  With
-// PREFETCH the list quads of batch k+1 are issued right after the gathers of
-// batch k, so a batch costs one memory round trip instead of two; that pays for
-// 16-byte gathers and not for the 32-byte (Pair4) ones.  Slots past c are never
-// used (their gathers load the particle itself, `self`, always a valid index).
+// batch's list quads, then its NB dependent gathers, then the arithmetic.  The
+// gathers are latency-bound, so the depth NB (gathers in flight per wave) sets
+// the speed: 16 was 13-22% faster than 8 in every gather pass on MI355X; the
+// autotuner chooses 8, 16 or 32 per kernel.  With PREFETCH the list quads of
+// batch k+1 are issued right after the gathers of batch k, so a batch costs
+// one memory round trip instead of two; that pays for 16-byte gathers and not
+// for the 32-byte (Pair4) ones.  Slots past c are never used; their gathers
+// load a valid index (the particle itself in a wide list, the layer-2 base in
+// a compact one, which the build sets to the particle itself when that layer
+// has no candidates).
 constexpr int kBatch = 8;  // candidate batch of the list build
 constexpr uint32_t kQuadStride = kWave * 4;  // list words between a lane's quads
 
 template <int NB>
-__device__ __forceinline__ void load_quads(const uint32_t *L, int k0, int c, uint32_t self,
-                                           uint32_t (&j)[NB]) {
-  const uint4 *Q = reinterpret_cast<const uint4 *>(L);
+__device__ __forceinline__ void load_wide(const NbList &nl, int k0, uint32_t self,
+                                          uint32_t (&j)[NB]) {
+  const uint4 *Q = reinterpret_cast<const uint4 *>(nl.L);
   const int q0 = k0 >> 2;
 #pragma unroll
   for (int u = 0; u < NB / 4; ++u) {
     uint4 a = make_uint4(self, self, self, self);
-    if (k0 + 4 * u < c) a = Q[(size_t)(q0 + u) * kWave];
+    if (k0 + 4 * u < nl.c) a = Q[(size_t)(q0 + u) * kWave];
     j[4 * u] = a.x;
     j[4 * u + 1] = a.y;
     j[4 * u + 2] = a.z;
@@ -65,24 +93,61 @@ __device__ __forceinline__ void load_quads(const uint32_t *L, int k0, int c, uin
   }
 #pragma unroll
   for (int t = 0; t < NB; ++t)
-    if (k0 + t >= c) j[t] = self;
+    if (k0 + t >= nl.c) j[t] = self;
 }
 
-template <typename T, int NB, bool PREFETCH = true, typename Load, typename Body>
-__device__ __forceinline__ void for_neighbors(const uint32_t *L, int c, uint32_t self, Load load,
-                                              Body body) {
+// Compact quads: entry k = base of its layer + 16-bit offset; padding entries
+// are offset 0 from the layer-2 base.
+template <int NB>
+__device__ __forceinline__ void load_compact(const NbList &nl, int k0, uint32_t (&j)[NB]) {
+  static_assert(NB % 8 == 0, "compact quads hold 8 entries");
+  const uint4 *Q = reinterpret_cast<const uint4 *>(nl.L);
+  const int q0 = k0 >> 3;
+#pragma unroll
+  for (int u = 0; u < NB / 8; ++u) {
+    uint4 a = make_uint4(0u, 0u, 0u, 0u);
+    if (k0 + 8 * u < nl.c) a = Q[(size_t)(q0 + u) * kWave];
+    const uint32_t w[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int k = k0 + 8 * u + t;
+      const uint32_t off = (t & 1) ? (w[t >> 1] >> 16) : (w[t >> 1] & 0xFFFFu);
+      const uint32_t b = k < nl.e0 ? nl.h.x : (k < nl.e1 ? nl.h.y : nl.h.z);
+      j[8 * u + t] = b + off;
+    }
+  }
+}
+
+template <typename T, int NB, bool PREFETCH, bool WIDE, typename Load, typename Body>
+__device__ __forceinline__ void gather_loop(const NbList &nl, uint32_t self, Load &load,
+                                            Body &body) {
   uint32_t jn[NB];
-  load_quads<NB>(L, 0, c, self, jn);
-  for (int k0 = 0; k0 < c; k0 += NB) {
-    if (!PREFETCH && k0 > 0) load_quads<NB>(L, k0, c, self, jn);
+  auto fetch = [&](int k0) {
+    if (WIDE)
+      load_wide<NB>(nl, k0, self, jn);
+    else
+      load_compact<NB>(nl, k0, jn);
+  };
+  fetch(0);
+  for (int k0 = 0; k0 < nl.c; k0 += NB) {
+    if (!PREFETCH && k0 > 0) fetch(k0);
     T q[NB];
 #pragma unroll
     for (int b = 0; b < NB; ++b) q[b] = load(jn[b]);
-    if (PREFETCH) load_quads<NB>(L, k0 + NB, c, self, jn);
+    if (PREFETCH) fetch(k0 + NB);
 #pragma unroll
     for (int b = 0; b < NB; ++b)
-      if (k0 + b < c) body(q[b]);
+      if (k0 + b < nl.c) body(q[b]);
   }
+}
+
+template <typename T, int NB, bool PREFETCH = true, typename Load, typename Body>
+__device__ __forceinline__ void for_neighbors(const NbList &nl, uint32_t self, Load load,
+                                              Body body) {
+  if (nl.wide)
+    gather_loop<T, NB, PREFETCH, true>(nl, self, load, body);
+  else
+    gather_loop<T, NB, PREFETCH, false>(nl, self, load, body);
 }
 
 struct Pair4 {
@@ -91,6 +156,12 @@ struct Pair4 {
 
 __device__ __forceinline__ float fast_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// x^2 + y^2 + z^2 with two fused multiply-adds (this file is compiled without
+// implicit contraction, so every fma is written out).
+__device__ __forceinline__ float sq3(float x, float y, float z) {
+  return fmaf(z, z, fmaf(y, y, x * x));
+}
 
 // ApplyBoundaryConditions (particles.c:842-867).
 __device__ __forceinline__ float clamp_axis(float x, float lo, float hi, float hi_clamp) {
@@ -305,27 +376,27 @@ __device__ __noinline__ bool inside_exact(float px, float py, float pz, float qx
 // indices; walking rows in (dz, dy) order and indices upward reproduces the
 // reference's (dz, dy, dx, id) order and therefore its 60-cap truncation.
 //
-// Hits are buffered in LDS as 16 bits (dz layer, offset from the layer's
-// first index) and written out after the scan: a wave then stores one
-// 256-byte row of the slot-major list per instruction.  Written as found,
-// lanes store their k-th neighbour at different times and partially written
-// lines are written back several times.  A particle whose dz layer spans
-// more than kLayerSpan indices (never near rest density: a layer holds about
-// two grid rows of particles) writes its hits directly.
-constexpr uint32_t kLayerSpan = 1u << 14;
-
+// Hits are buffered in LDS as 16-bit offsets from the first candidate of
+// their dz layer and written out after the scan as compact quads (8 entries
+// each): a wave then stores 1 KB of the list contiguously per instruction.
+// Written as found, lanes store their k-th neighbour at different times and
+// partially written lines are written back several times.  A wavefront in
+// which some particle's layer spans more than kCompactSpan indices (never
+// near rest density: a layer holds about two grid rows of particles), or
+// every wavefront when `force_wide` is set, writes 32-bit indices directly.
+// The head records the layer bases (the particle itself for a layer without
+// candidates, so that padding entries decode to a valid index) and counts.
 template <bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_build_lists(
     uint32_t n, uint32_t nblocks, const float4 *__restrict__ pos,
     const uint32_t *__restrict__ key_s, const uint32_t *__restrict__ cstart, DevParams P,
-    uint32_t *__restrict__ list, int *__restrict__ cnt, uint32_t *__restrict__ stats,
-    uint32_t *__restrict__ pair_partial, BatchDev B) {
+    uint32_t *__restrict__ list, uint4 *__restrict__ head, uint32_t *__restrict__ stats,
+    uint32_t *__restrict__ pair_partial, int force_wide, BatchDev B) {
   extern __shared__ uint16_t sbuf[];  // [kMaxNb][blockDim.x]
   const uint32_t s = particle_index(nblocks);
   int c = 0;
   uint32_t dropped = 0;
-  bool direct = false;
-  uint32_t base[3] = {0u, 0u, 0u};
+  bool wide = false;
   uint32_t *out = list + (size_t)(s >> 6) * kListStride + (s & 63u) * 4u;
   if (s < n) {
     const float4 p = pos[s];
@@ -339,49 +410,71 @@ __global__ __launch_bounds__(kBlock) void k_build_lists(
     const uint32_t x1 = cx + 1 < G.nx ? cx + 1 : G.nx - 1;
     const uint32_t y0 = cy > 0 ? cy - 1 : 0;
     const uint32_t y1 = cy + 1 < G.ny ? cy + 1 : G.ny - 1;
+    uint32_t base[3] = {s, s, s};
+    bool direct = force_wide != 0;
 #pragma unroll
     for (int l = 0; l < 3; ++l) {
       const int z = (int)cz + l - 1;
       if (z < 0 || z >= (int)G.nz) continue;
       const uint32_t zrow = G.base + (uint32_t)z * G.nxy;
-      base[l] = cstart[zrow + y0 * G.nx + x0];
-      if (cstart[zrow + y1 * G.nx + x1 + 1] - base[l] > kLayerSpan) direct = true;
+      const uint32_t first = cstart[zrow + y0 * G.nx + x0];
+      const uint32_t span = cstart[zrow + y1 * G.nx + x1 + 1] - first;
+      if (span > 0u) base[l] = first;
+      if (span > kCompactSpan) direct = true;
     }
+    // the format is per wavefront (the lanes' quads interleave)
+    wide = __ballot(direct) != 0ull;
+    int cl[3] = {0, 0, 0};
     for (int dz = -1; dz <= 1; ++dz) {
       const int z = (int)cz + dz;
-      if (z < 0 || z >= (int)G.nz) continue;
-      const uint32_t lb = dz < 0 ? base[0] : (dz == 0 ? base[1] : base[2]);
-      const uint32_t tag = (uint32_t)(dz + 1) << 14;
-      for (int dy = -1; dy <= 1; ++dy) {
-        const int y = (int)cy + dy;
-        if (y < 0 || y >= (int)G.ny) continue;
-        const uint32_t row = G.base + (uint32_t)z * G.nxy + (uint32_t)y * G.nx;
-        const uint32_t j0 = cstart[row + x0];
-        const uint32_t j1 = cstart[row + x1 + 1];
-        // candidates in batches of kBatch loads; hits are appended in j order
-        for (uint32_t jb = j0; jb < j1; jb += kBatch) {
-          float4 q[kBatch];
+      if (z >= 0 && z < (int)G.nz) {
+        const uint32_t lb = dz < 0 ? base[0] : (dz == 0 ? base[1] : base[2]);
+        for (int dy = -1; dy <= 1; ++dy) {
+          const int y = (int)cy + dy;
+          if (y < 0 || y >= (int)G.ny) continue;
+          const uint32_t row = G.base + (uint32_t)z * G.nxy + (uint32_t)y * G.nx;
+          const uint32_t j0 = cstart[row + x0];
+          const uint32_t j1 = cstart[row + x1 + 1];
+          // candidates 32 at a time: test all (8 loads in flight per group),
+          // then append the hits of the chunk in j order
+          for (uint32_t cb = j0; cb < j1; cb += 32) {
+            const uint32_t len = j1 - cb;
+            uint32_t lo = 0u, hi = 0u;  // bits: r2 < h2_lo, r2 < h2_hi
 #pragma unroll
-          for (int b = 0; b < kBatch; ++b) q[b] = pos[(jb + b < j1) ? jb + b : s];
+            for (int g = 0; g < 4; ++g) {
+              if (8u * g < len) {
+                // reads up to 7 entries past j1: the position arrays are
+                // padded by kPosPad readable entries beyond the capacity
+                const float4 *Q = pos + cb + 8 * g;
+                float4 q[kBatch];
 #pragma unroll
-          for (int b = 0; b < kBatch; ++b) {
-            const uint32_t j = jb + b;
-            if (j >= j1 || j == s) continue;
-            const float ddx = p.x - q[b].x, ddy = p.y - q[b].y, ddz = p.z - q[b].z;
-            const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-            bool hit;
-            if (r2 < P.h2_lo)
-              hit = true;
-            else if (r2 >= P.h2_hi)
-              hit = false;
-            else
-              hit = inside_exact(p.x, p.y, p.z, q[b].x, q[b].y, q[b].z, P.h2d);
-            if (hit) {
+                for (int t = 0; t < kBatch; ++t) q[t] = Q[t];
+#pragma unroll
+                for (int t = 0; t < kBatch; ++t) {
+                  const float r2 = sq3(p.x - q[t].x, p.y - q[t].y, p.z - q[t].z);
+                  lo |= (r2 < P.h2_lo ? 1u : 0u) << (8 * g + t);
+                  hi |= (r2 < P.h2_hi ? 1u : 0u) << (8 * g + t);
+                }
+              }
+            }
+            uint32_t keep = len >= 32u ? ~0u : (1u << len) - 1u;
+            if (s - cb < 32u) keep &= ~(1u << (s - cb));  // not the particle itself
+            lo &= keep;
+            uint32_t band = hi & keep & ~lo;  // within the guard band of h^2
+            while (band) {
+              const int t = __builtin_ctz(band);
+              band &= band - 1u;
+              const float4 q = pos[cb + t];
+              if (inside_exact(p.x, p.y, p.z, q.x, q.y, q.z, P.h2d)) lo |= 1u << t;
+            }
+            while (lo) {
+              const uint32_t j = cb + (uint32_t)__builtin_ctz(lo);
+              lo &= lo - 1u;
               if (c < kMaxNb) {
-                if (direct)
+                if (wide)
                   out[(size_t)(c >> 2) * kQuadStride + (c & 3)] = j;
                 else
-                  sbuf[c * blockDim.x + threadIdx.x] = (uint16_t)(tag | (j - lb));
+                  sbuf[c * blockDim.x + threadIdx.x] = (uint16_t)(j - lb);
                 ++c;
               } else {
                 ++dropped;
@@ -390,26 +483,29 @@ __global__ __launch_bounds__(kBlock) void k_build_lists(
           }
         }
       }
+      cl[dz + 1] = c;
     }
-    cnt[s] = c;
+    head[s] = make_uint4(base[0], base[1], base[2],
+                         (uint32_t)c | ((uint32_t)cl[0] << 8) | ((uint32_t)cl[1] << 16) |
+                             (wide ? kHeadWide : 0u));
   }
-  // coalesced write-out: every lane of the wave walks the wave's longest list
-  // one quad at a time; a wave store covers 1 KB of the list contiguously
-  int cmax = direct ? 0 : c;
+  // coalesced write-out of compact quads: every lane of the wave walks the
+  // wave's longest list one quad (8 entries) at a time
+  int cmax = wide ? 0 : c;
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) cmax = max(cmax, __shfl_xor(cmax, o, 64));
-  for (int k = 0; k < cmax; k += 4) {
-    if (k < c && !direct) {
-      uint32_t v[4];
+  for (int k = 0; k < cmax; k += 8) {
+    if (k < c) {
+      uint32_t w[4];
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        const uint32_t e = (k + t < c) ? sbuf[(k + t) * blockDim.x + threadIdx.x] : 0u;
-        const uint32_t l = e >> 14;
-        const uint32_t bb = l == 0 ? base[0] : (l == 1 ? base[1] : base[2]);
-        v[t] = (k + t < c) ? bb + (e & (kLayerSpan - 1u)) : s;
+        const uint32_t lo = (k + 2 * t < c) ? sbuf[(k + 2 * t) * blockDim.x + threadIdx.x] : 0u;
+        const uint32_t hi =
+            (k + 2 * t + 1 < c) ? sbuf[(k + 2 * t + 1) * blockDim.x + threadIdx.x] : 0u;
+        w[t] = lo | (hi << 16);
       }
-      *reinterpret_cast<uint4 *>(out + (size_t)(k >> 2) * kQuadStride) =
-          make_uint4(v[0], v[1], v[2], v[3]);
+      *reinterpret_cast<uint4 *>(out + (size_t)(k >> 3) * kQuadStride) =
+          make_uint4(w[0], w[1], w[2], w[3]);
     }
   }
   add_counter64(stats, STAT_DROPPED, dropped);
@@ -445,7 +541,7 @@ __device__ __forceinline__ bool converged(const uint32_t *stats) {
 template <int MODE, int NB>
 __global__ __launch_bounds__(kBlock) void k_density_lambda(
     uint32_t n, uint32_t nblocks, float4 *pos, const uint32_t *__restrict__ list,
-    const int *__restrict__ cnt, DevParams P, float *rho_io, uint32_t *stats, int err_slot,
+    const uint4 *__restrict__ head, DevParams P, float *rho_io, uint32_t *stats, int err_slot,
     int check_converged, int density_to_w) {
   if (check_converged && converged(stats)) return;
   const uint32_t s = particle_index(nblocks);
@@ -453,30 +549,29 @@ __global__ __launch_bounds__(kBlock) void k_density_lambda(
   float err = 0.0f;
   if (active) {
     const float4 p = pos[s];
-    const int c = cnt[s];
-    const uint32_t *L = list_ptr(list, s);
+    const NbList nl = nb_list(list, head, s);
     float su3 = 0.0f, sgx = 0.0f, sgy = 0.0f, sgz = 0.0f, sc = 0.0f;
-    for_neighbors<float4, NB>(L, c, s, [&](uint32_t j) { return pos[j]; }, [&](const float4 &q) {
+    for_neighbors<float4, NB>(nl, s, [&](uint32_t j) { return pos[j]; }, [&](const float4 &q) {
       const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-      const float r2 = dx * dx + dy * dy + dz * dz;
+      const float r2 = sq3(dx, dy, dz);
       if (MODE != 2) {
-        const float u = P.h2 - r2;  // W(r) = norm (h^2 - r^2)^3, 0 for r >= h
-        su3 += u > 0.0f ? u * u * u : 0.0f;
+        const float u = fmaxf(P.h2 - r2, 0.0f);  // W(r) = norm (h^2 - r^2)^3, 0 for r >= h
+        su3 = fmaf(u * u, u, su3);
       }
       if (MODE != 1) {
         const float r2c = fmaxf(r2, P.rmin2);
         const float ir = fast_rsq(r2c);
-        const float hr = P.h - r2c * ir;  // h - r, r floored at 1e-4 h
+        const float hr = fmaf(-r2c, ir, P.h);  // h - r, r floored at 1e-4 h
         const float g = hr > 0.0f ? hr * hr * ir : 0.0f;  // DelW(r)/r / norm
-        sgx += g * dx;
-        sgy += g * dy;
-        sgz += g * dz;
-        sc += g * g * r2;
+        sgx = fmaf(g, dx, sgx);
+        sgy = fmaf(g, dy, sgy);
+        sgz = fmaf(g, dz, sgz);
+        sc = fmaf(g * g, r2, sc);
       }
     });
     float rho;
     if (MODE != 2) {
-      rho = P.mW0 + P.mWn * su3;
+      rho = fmaf(P.mWn, su3, P.mW0);
       rho_io[s] = rho;
       err = fmaxf((rho - P.rest_density) * P.inv_rest_density, 0.0f);
       if (density_to_w) pos[s].w = rho;
@@ -485,7 +580,7 @@ __global__ __launch_bounds__(kBlock) void k_density_lambda(
     }
     if (MODE != 1) {
       const float Ci = fmaxf((rho - P.rest_density) * P.inv_rest_density, 0.0f);
-      const float sum_c = P.lam_k2 * (sc + sgx * sgx + sgy * sgy + sgz * sgz);
+      const float sum_c = P.lam_k2 * (sc + sq3(sgx, sgy, sgz));
       const float lam = -Ci / (sum_c + P.lambda_eps);
       pos[s].w = lam;
     }
@@ -543,20 +638,19 @@ __device__ __forceinline__ void collide(float &x, float &y, float &z, const SdfD
 
 // UpdateDPs (particles.c:668-746): returns sum_j (lambda_i + lambda_j) DelW(r)/r d.
 template <int NB>
-__device__ __forceinline__ float3 dp_sum(const float4 &p, const float4 *pos, const uint32_t *L,
-                                         int c, uint32_t self, const DevParams &P) {
+__device__ __forceinline__ float3 dp_sum(const float4 &p, const float4 *pos, const NbList &nl,
+                                         uint32_t self, const DevParams &P) {
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
-  for_neighbors<float4, NB>(L, c, self, [&](uint32_t j) { return pos[j]; }, [&](const float4 &q) {
+  for_neighbors<float4, NB>(nl, self, [&](uint32_t j) { return pos[j]; }, [&](const float4 &q) {
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-    const float r2 = dx * dx + dy * dy + dz * dz;
-    const float r2c = fmaxf(r2, P.rmin2);
+    const float r2c = fmaxf(sq3(dx, dy, dz), P.rmin2);
     const float ir = fast_rsq(r2c);
-    const float hr = P.h - r2c * ir;
+    const float hr = fmaf(-r2c, ir, P.h);
     const float g = hr > 0.0f ? hr * hr * ir : 0.0f;
     const float f = (p.w + q.w) * g;
-    ax += f * dx;
-    ay += f * dy;
-    az += f * dz;
+    ax = fmaf(f, dx, ax);
+    ay = fmaf(f, dy, ay);
+    az = fmaf(f, dz, az);
   });
   return make_float3(ax, ay, az);
 }
@@ -566,12 +660,12 @@ template <int NB>
 __global__ __launch_bounds__(kBlock) void k_dp(uint32_t n, uint32_t nblocks,
                                                const float4 *__restrict__ pos,
                                                const uint32_t *__restrict__ list,
-                                               const int *__restrict__ cnt, DevParams P,
+                                               const uint4 *__restrict__ head, DevParams P,
                                                float scale, float4 *__restrict__ dp) {
   const uint32_t s = particle_index(nblocks);
   if (s >= n) return;
   const float4 p = pos[s];
-  const float3 a = dp_sum<NB>(p, pos, list_ptr(list, s), cnt[s], s, P);
+  const float3 a = dp_sum<NB>(p, pos, nb_list(list, head, s), s, P);
   dp[s] = make_float4(scale * a.x, scale * a.y, scale * a.z, 0.0f);
 }
 
@@ -602,7 +696,7 @@ template <bool COLLIDE, bool BATCH, int NB>
 __global__ __launch_bounds__(kBlock) void k_dp_update(
     uint32_t n, uint32_t nblocks, const float4 *__restrict__ pin, float4 *__restrict__ pout,
     const float *__restrict__ rho, const uint32_t *__restrict__ list,
-    const int *__restrict__ cnt, DevParams P, float scale, SdfDev o, uint32_t *stats,
+    const uint4 *__restrict__ head, DevParams P, float scale, SdfDev o, uint32_t *stats,
     int err_slot, float tol, float *__restrict__ lam_acc, float stiff, BatchDev B) {
   bool done = false;
   if (tol > 0.0f) {
@@ -618,12 +712,12 @@ __global__ __launch_bounds__(kBlock) void k_dp_update(
     return;
   }
   if (s == 0) stats[STAT_ITERS] += 1u;
-  const float3 a = dp_sum<NB>(p, pin, list_ptr(list, s), cnt[s], s, P);
-  float x = p.x + scale * a.x, y = p.y + scale * a.y, z = p.z + scale * a.z;
+  const float3 a = dp_sum<NB>(p, pin, nb_list(list, head, s), s, P);
+  float x = fmaf(scale, a.x, p.x), y = fmaf(scale, a.y, p.y), z = fmaf(scale, a.z, p.z);
   apply_boundary(x, y, z, geo_of<BATCH>(s, P, B), P);
   if (COLLIDE) collide(x, y, z, sdf_of<BATCH>(s, o, B));
   pout[s] = make_float4(x, y, z, rho[s]);
-  if (lam_acc) lam_acc[s] += stiff * p.w;
+  if (lam_acc) lam_acc[s] = fmaf(stiff, p.w, lam_acc[s]);
 }
 
 // ---------------------------------------------------------------------------
@@ -667,7 +761,7 @@ __device__ __forceinline__ void sdf_features(float x, float y, float z, const Sd
 template <bool LOG, bool BATCH, int NB = 8>
 __global__ __launch_bounds__(kBlock) void k_warm_lambda(
     uint32_t n, uint32_t nblocks, float4 *pos, const float4 *__restrict__ xold,
-    const uint32_t *__restrict__ list, const int *__restrict__ cnt, DevParams P,
+    const uint32_t *__restrict__ list, const uint4 *__restrict__ head, DevParams P,
     float *__restrict__ rho_io, uint32_t *stats, const uint32_t *__restrict__ id_s,
     const float *__restrict__ lam_prev, SdfDev o, const MlpPacked *__restrict__ net,
     float *__restrict__ lam_acc, float *__restrict__ feat, float2 *__restrict__ plain,
@@ -682,39 +776,39 @@ __global__ __launch_bounds__(kBlock) void k_warm_lambda(
     const float4 p = pos[s];
     const float4 po = xold[s];
     const float ux = p.x - po.x, uy = p.y - po.y, uz = p.z - po.z;  // dt * v_i
-    const int c = cnt[s];
+    const NbList nl = nb_list(list, head, s);
     float su3 = 0.0f, sgx = 0.0f, sgy = 0.0f, sgz = 0.0f, sc = 0.0f;
     float mx = 0.0f, my = 0.0f, mz = 0.0f, dv = 0.0f;
     auto load = [&](uint32_t j) { return Pair4{pos[j], xold[j]}; };
-    for_neighbors<Pair4, NB, false>(list_ptr(list, s), c, s, load, [&](const Pair4 &qq) {
+    for_neighbors<Pair4, NB, false>(nl, s, load, [&](const Pair4 &qq) {
       const float4 &q = qq.a, &qo = qq.b;
       const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-      const float r2 = dx * dx + dy * dy + dz * dz;
-      const float u = P.h2 - r2;
-      const float w3 = u > 0.0f ? u * u * u : 0.0f;
-      su3 += w3;
-      mx -= w3 * dx;
-      my -= w3 * dy;
-      mz -= w3 * dz;
+      const float r2 = sq3(dx, dy, dz);
+      const float u = fmaxf(P.h2 - r2, 0.0f);
+      const float w3 = u * u * u;
+      su3 = fmaf(u * u, u, su3);  // as k_density_lambda accumulates it
+      mx = fmaf(-w3, dx, mx);
+      my = fmaf(-w3, dy, my);
+      mz = fmaf(-w3, dz, mz);
       const float r2c = fmaxf(r2, P.rmin2);
       const float ir = fast_rsq(r2c);
-      const float hr = P.h - r2c * ir;
+      const float hr = fmaf(-r2c, ir, P.h);
       const float g = hr > 0.0f ? hr * hr * ir : 0.0f;
-      sgx += g * dx;
-      sgy += g * dy;
-      sgz += g * dz;
-      sc += g * g * r2;
+      sgx = fmaf(g, dx, sgx);
+      sgy = fmaf(g, dy, sgy);
+      sgz = fmaf(g, dz, sgz);
+      sc = fmaf(g * g, r2, sc);
       const float ex = (q.x - qo.x) - ux, ey = (q.y - qo.y) - uy, ez = (q.z - qo.z) - uz;
-      dv += g * (ex * dx + ey * dy + ez * dz);
+      dv = fmaf(g, fmaf(ez, dz, fmaf(ey, dy, ex * dx)), dv);
     });
-    rho = P.mW0 + P.mWn * su3;
+    rho = fmaf(P.mWn, su3, P.mW0);
     rho_io[s] = rho;
     const float cr = (rho - P.rest_density) * P.inv_rest_density;
     err = fmaxf(cr, 0.0f);
-    const float gc2 = P.lam_k2 * (sc + sgx * sgx + sgy * sgy + sgz * sgz);
+    const float gc2 = P.lam_k2 * (sc + sq3(sgx, sgy, sgz));
     denom = gc2 + P.lambda_eps;
     lam0 = -err / denom;
-    f[0] = (float)c * (1.0f / 32.0f);
+    f[0] = (float)nl.c * (1.0f / 32.0f);
     f[1] = cr;
     f[2] = -lam_prev[id_s[s]] * denom;
     f[3] = gc2 / denom;
@@ -835,24 +929,23 @@ __global__ __launch_bounds__(kBlock) void k_color(uint32_t n, uint32_t nblocks,
                                                   const float4 *__restrict__ pos,
                                                   const float4 *__restrict__ xold,
                                                   const uint32_t *__restrict__ list,
-                                                  const int *__restrict__ cnt, DevParams P,
+                                                  const uint4 *__restrict__ head, DevParams P,
                                                   float4 *__restrict__ vel,
                                                   float4 *__restrict__ color) {
   const uint32_t s = particle_index(nblocks);
   if (s >= n) return;
   const float4 p = pos[s];
   if (WITH_VELOCITY) vel[s] = velocity_of(p, xold[s], P);
-  const uint32_t *L = list_ptr(list, s);
-  const int c = cnt[s];
+  const NbList nl = nb_list(list, head, s);
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
-  for_neighbors<float4, NB>(L, c, s, [&](uint32_t j) { return pos[j]; }, [&](const float4 &q) {
+  for_neighbors<float4, NB>(nl, s, [&](uint32_t j) { return pos[j]; }, [&](const float4 &q) {
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-    const float r2c = fmaxf(dx * dx + dy * dy + dz * dz, P.rmin2);
+    const float r2c = fmaxf(sq3(dx, dy, dz), P.rmin2);
     const float u = P.h2 - r2c;
     const float g = u > 0.0f ? u * u * fast_rcp(q.w) : 0.0f;
-    ax += g * dx;
-    ay += g * dy;
-    az += g * dz;
+    ax = fmaf(g, dx, ax);
+    ay = fmaf(g, dy, ay);
+    az = fmaf(g, dz, az);
   });
   color[s] = make_float4(P.color_k * ax, P.color_k * ay, P.color_k * az, 0.0f);
 }
@@ -864,40 +957,39 @@ __global__ __launch_bounds__(kBlock) void k_surface(uint32_t n, uint32_t nblocks
                                                     const float4 *__restrict__ pos,
                                                     const float4 *__restrict__ color,
                                                     const uint32_t *__restrict__ list,
-                                                    const int *__restrict__ cnt, DevParams P,
+                                                    const uint4 *__restrict__ head, DevParams P,
                                                     float4 *__restrict__ vel) {
   const uint32_t s = particle_index(nblocks);
   if (s >= n) return;
   const float4 p = pos[s];
   const float4 ci = color[s];
-  const uint32_t *L = list_ptr(list, s);
-  const int c = cnt[s];
+  const NbList nl = nb_list(list, head, s);
   const float hh = 0.5f * P.h;
   float fx = 0.0f, fy = 0.0f, fz = 0.0f;
   auto load = [&](uint32_t j) { return Pair4{pos[j], color[j]}; };
-  for_neighbors<Pair4, NB, false>(L, c, s, load, [&](const Pair4 &qc) {
+  for_neighbors<Pair4, NB, false>(nl, s, load, [&](const Pair4 &qc) {
     const float4 &q = qc.a, &cq = qc.b;
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-    const float r2c = fmaxf(dx * dx + dy * dy + dz * dz, P.rmin2);
+    const float r2c = fmaxf(sq3(dx, dy, dz), P.rmin2);
     const float ir = fast_rsq(r2c);
     const float r = r2c * ir;
     float spline = 0.0f;
     if (r <= P.h) {
       const float t = (P.h - r) * r;
       const float t3 = t * t * t;
-      spline = r <= hh ? 2.0f * t3 - P.h6_64 : t3;
+      spline = r <= hh ? fmaf(2.0f, t3, -P.h6_64) : t3;
     }
     const float coh = P.coh_k * spline * ir;
     const float K = 2.0f * P.rest_density * fast_rcp(p.w + q.w);
-    fx += K * (coh * dx + P.curv_k * (ci.x - cq.x));
-    fy += K * (coh * dy + P.curv_k * (ci.y - cq.y));
-    fz += K * (coh * dz + P.curv_k * (ci.z - cq.z));
+    fx = fmaf(K, fmaf(coh, dx, P.curv_k * (ci.x - cq.x)), fx);
+    fy = fmaf(K, fmaf(coh, dy, P.curv_k * (ci.y - cq.y)), fy);
+    fz = fmaf(K, fmaf(coh, dz, P.curv_k * (ci.z - cq.z)), fz);
   });
   const float k = P.dt / p.w;
   float4 v = vel[s];
-  v.x += k * fx;
-  v.y += k * fy;
-  v.z += k * fz;
+  v.x = fmaf(k, fx, v.x);
+  v.y = fmaf(k, fy, v.y);
+  v.z = fmaf(k, fz, v.z);
   vel[s] = v;
 }
 
@@ -908,27 +1000,26 @@ __global__ __launch_bounds__(kBlock) void k_xsph(uint32_t n, uint32_t nblocks,
                                                  const float4 *__restrict__ pos,
                                                  const float4 *__restrict__ vin,
                                                  const uint32_t *__restrict__ list,
-                                                 const int *__restrict__ cnt, DevParams P,
+                                                 const uint4 *__restrict__ head, DevParams P,
                                                  float4 *__restrict__ vout) {
   const uint32_t s = particle_index(nblocks);
   if (s >= n) return;
   const float4 p = pos[s];
   const float4 vi = vin[s];
-  const uint32_t *L = list_ptr(list, s);
-  const int c = cnt[s];
+  const NbList nl = nb_list(list, head, s);
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
   auto load = [&](uint32_t j) { return Pair4{pos[j], vin[j]}; };
-  for_neighbors<Pair4, NB, false>(L, c, s, load, [&](const Pair4 &qv) {
+  for_neighbors<Pair4, NB, false>(nl, s, load, [&](const Pair4 &qv) {
     const float4 &q = qv.a, &vq = qv.b;
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-    const float u = P.h2 - (dx * dx + dy * dy * dz * dz);
+    const float u = P.h2 - fmaf(dy * dy * dz, dz, dx * dx);
     const float w = u > 0.0f ? u * u * u * fast_rcp(q.w) : 0.0f;
-    ax += (vq.x - vi.x) * w;
-    ay += (vq.y - vi.y) * w;
-    az += (vq.z - vi.z) * w;
+    ax = fmaf(vq.x - vi.x, w, ax);
+    ay = fmaf(vq.y - vi.y, w, ay);
+    az = fmaf(vq.z - vi.z, w, az);
   });
   const float k = P.xsph_c * P.mWn;
-  vout[s] = make_float4(vi.x + k * ax, vi.y + k * ay, vi.z + k * az, 0.0f);
+  vout[s] = make_float4(fmaf(k, ax, vi.x), fmaf(k, ay, vi.y), fmaf(k, az, vi.z), 0.0f);
 }
 
 // ComputeVorticity (particles.c:207-283); aux = (omega, |omega|), and
@@ -938,36 +1029,35 @@ __global__ __launch_bounds__(kBlock) void k_vorticity(uint32_t n, uint32_t nbloc
                                                       const float4 *__restrict__ pos,
                                                       const float4 *__restrict__ vel,
                                                       const uint32_t *__restrict__ list,
-                                                      const int *__restrict__ cnt, DevParams P,
+                                                      const uint4 *__restrict__ head, DevParams P,
                                                       float4 *__restrict__ omega,
                                                       float4 *__restrict__ pw) {
   const uint32_t s = particle_index(nblocks);
   if (s >= n) return;
   const float4 p = pos[s];
   const float4 vi = vel[s];
-  const uint32_t *L = list_ptr(list, s);
-  const int c = cnt[s];
+  const NbList nl = nb_list(list, head, s);
   float ox = 0.0f, oy = 0.0f, oz = 0.0f;
   auto load = [&](uint32_t j) { return Pair4{pos[j], vel[j]}; };
-  for_neighbors<Pair4, NB, false>(L, c, s, load, [&](const Pair4 &qv) {
+  for_neighbors<Pair4, NB, false>(nl, s, load, [&](const Pair4 &qv) {
     const float4 &q = qv.a, &vq = qv.b;
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-    const float r2c = fmaxf(dx * dx + dy * dy + dz * dz, P.rmin2);
+    const float r2c = fmaxf(sq3(dx, dy, dz), P.rmin2);
     const float ir = fast_rsq(r2c);
-    const float hr = P.h - r2c * ir;
+    const float hr = fmaf(-r2c, ir, P.h);
     float g = hr > 0.0f ? hr * hr * ir : 0.0f;
     if (P.vort_model == 1) g *= fast_rcp(q.w);
     const float wx = g * dx, wy = g * dy, wz = g * dz;
     const float vx = vq.x - vi.x, vy = vq.y - vi.y, vz = vq.z - vi.z;
-    ox += vy * wz - vz * wy;
-    oy += vz * wx - vx * wz;
-    oz += vx * wy - vy * wx;
+    ox = fmaf(vy, wz, fmaf(-vz, wy, ox));
+    oy = fmaf(vz, wx, fmaf(-vx, wz, oy));
+    oz = fmaf(vx, wy, fmaf(-vy, wx, oz));
   });
   const float k = P.vort_model == 1 ? P.DelW_norm * P.mass : P.DelW_norm;
   ox *= k;
   oy *= k;
   oz *= k;
-  const float wn = sqrtf(ox * ox + oy * oy + oz * oz);
+  const float wn = sqrtf(sq3(ox, oy, oz));
   omega[s] = make_float4(ox, oy, oz, wn);
   pw[s] = make_float4(p.x, p.y, p.z, wn);
 }
@@ -980,30 +1070,29 @@ template <bool COMMIT, int NB>
 __global__ __launch_bounds__(kBlock) void k_confine(
     uint32_t n, uint32_t nblocks, const float4 *__restrict__ pw,
     const float4 *__restrict__ omega, const uint32_t *__restrict__ list,
-    const int *__restrict__ cnt, DevParams P, const float4 *__restrict__ vin,
+    const uint4 *__restrict__ head, DevParams P, const float4 *__restrict__ vin,
     float4 *__restrict__ vout, const uint32_t *__restrict__ id_s, float4 *__restrict__ x_can,
     float4 *__restrict__ v_can) {
   const uint32_t s = particle_index(nblocks);
   if (s >= n) return;
   const float4 p = pw[s];
-  const uint32_t *L = list_ptr(list, s);
-  const int c = cnt[s];
+  const NbList nl = nb_list(list, head, s);
   float ex = 0.0f, ey = 0.0f, ez = 0.0f;
   // only x_j and |omega_j| of the neighbour are needed
-  for_neighbors<float4, NB>(L, c, s, [&](uint32_t j) { return pw[j]; }, [&](const float4 &q) {
+  for_neighbors<float4, NB>(nl, s, [&](uint32_t j) { return pw[j]; }, [&](const float4 &q) {
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-    const float r2c = fmaxf(dx * dx + dy * dy + dz * dz, P.rmin2);
+    const float r2c = fmaxf(sq3(dx, dy, dz), P.rmin2);
     const float ir = fast_rsq(r2c);
-    const float hr = P.h - r2c * ir;
+    const float hr = fmaf(-r2c, ir, P.h);
     const float g = hr > 0.0f ? q.w * hr * hr * ir : 0.0f;
-    ex += g * dx;
-    ey += g * dy;
-    ez += g * dz;
+    ex = fmaf(g, dx, ex);
+    ey = fmaf(g, dy, ey);
+    ez = fmaf(g, dz, ez);
   });
   ex *= P.DelW_norm;
   ey *= P.DelW_norm;
   ez *= P.DelW_norm;
-  const float em = fmaxf(sqrtf(ex * ex + ey * ey + ez * ez), P.rmin);
+  const float em = fmaxf(sqrtf(sq3(ex, ey, ez)), P.rmin);
   const float Nx = ex / em, Ny = ey / em, Nz = ez / em;
   const float4 wi = omega[s];
   float4 v = vin[s];
@@ -1235,17 +1324,17 @@ int launch_neighbor_search(sph_ctx *c, bool fused) {
                                                             c->pos, c->xold));
     SPH_HIP(hipGetLastError());
     const uint32_t bs = tuned_block(c, K_BUILD_LISTS);
-  const uint32_t nb = blocks_for(n, bs);
+    const uint32_t nb = blocks_for(n, bs);
     if (c->nscenes)
       LAUNCH_TIMED(c, K_BUILD_LISTS, 0,
                    k_build_lists<true><<<nb, bs, kMaxNb * bs * sizeof(uint16_t), s>>>(
-                       n, nb, c->pos, c->key_s, c->cell_start, c->dev, c->list, c->list_cnt,
-                       c->stats, c->pair_partial, B));
+                       n, nb, c->pos, c->key_s, c->cell_start, c->dev, c->list, c->list_head,
+                       c->stats, c->pair_partial, c->tune.wide_lists ? 1 : 0, B));
     else
       LAUNCH_TIMED(c, K_BUILD_LISTS, 0,
                    k_build_lists<false><<<nb, bs, kMaxNb * bs * sizeof(uint16_t), s>>>(
-                       n, nb, c->pos, c->key_s, c->cell_start, c->dev, c->list, c->list_cnt,
-                       c->stats, c->pair_partial, B));
+                       n, nb, c->pos, c->key_s, c->cell_start, c->dev, c->list, c->list_head,
+                       c->stats, c->pair_partial, c->tune.wide_lists ? 1 : 0, B));
     c->pair_blocks = nb;
     SPH_HIP(hipGetLastError());
   } else {
@@ -1274,7 +1363,7 @@ int launch_density(sph_ctx *c, int mode, int err_slot) {
     with_depth(tuned_depth(c, K_DENSITY_LAMBDA), [&](auto D_) {
       constexpr int NBv = decltype(D_)::value;
       LAUNCH_TIMED(c, K_DENSITY_LAMBDA, 0,
-                   k_density_lambda<0, NBv><<<nb, bs, 0, s>>>(n, nb, c->pos, c->list, c->list_cnt,
+                   k_density_lambda<0, NBv><<<nb, bs, 0, s>>>(n, nb, c->pos, c->list, c->list_head,
                                                              c->dev, c->rho, c->stats, err_slot,
                                                              c->P.density_tolerance > 0, 0));
     });
@@ -1282,7 +1371,7 @@ int launch_density(sph_ctx *c, int mode, int err_slot) {
     with_depth(tuned_depth(c, K_DENSITY), [&](auto D_) {
       constexpr int NBv = decltype(D_)::value;
       LAUNCH_TIMED(c, K_DENSITY, 0,
-                   k_density_lambda<1, NBv><<<nb, bs, 0, s>>>(n, nb, c->pos, c->list, c->list_cnt,
+                   k_density_lambda<1, NBv><<<nb, bs, 0, s>>>(n, nb, c->pos, c->list, c->list_head,
                                                              c->dev, c->rho, c->stats, err_slot, 0,
                                                              0));
     });
@@ -1290,7 +1379,7 @@ int launch_density(sph_ctx *c, int mode, int err_slot) {
     with_depth(tuned_depth(c, K_DENSITY), [&](auto D_) {
       constexpr int NBv = decltype(D_)::value;
       LAUNCH_TIMED(c, K_DENSITY, 0,
-                   k_density_lambda<1, NBv><<<nb, bs, 0, s>>>(n, nb, c->pos, c->list, c->list_cnt,
+                   k_density_lambda<1, NBv><<<nb, bs, 0, s>>>(n, nb, c->pos, c->list, c->list_head,
                                                              c->dev, c->rho, c->stats, err_slot, 1,
                                                              1));
     });
@@ -1298,7 +1387,7 @@ int launch_density(sph_ctx *c, int mode, int err_slot) {
     with_depth(tuned_depth(c, K_LAMBDA), [&](auto D_) {
       constexpr int NBv = decltype(D_)::value;
       LAUNCH_TIMED(c, K_LAMBDA, 0,
-                   k_density_lambda<2, NBv><<<nb, bs, 0, s>>>(n, nb, c->pos, c->list, c->list_cnt,
+                   k_density_lambda<2, NBv><<<nb, bs, 0, s>>>(n, nb, c->pos, c->list, c->list_head,
                                                              c->dev, c->rho, c->stats, -1, 0, 0));
     });
   SPH_HIP(hipGetLastError());
@@ -1313,7 +1402,7 @@ int launch_dp(sph_ctx *c, int substep) {
   with_depth(tuned_depth(c, K_DP), [&](auto D_) {
     constexpr int NBv = decltype(D_)::value;
     LAUNCH_TIMED(c, K_DP, 0,
-               k_dp<NBv><<<nb, bs, 0, c->stream>>>(n, nb, c->pos, c->list, c->list_cnt, c->dev,
+               k_dp<NBv><<<nb, bs, 0, c->stream>>>(n, nb, c->pos, c->list, c->list_head, c->dev,
                                                   stiffness_scale(c, substep), c->dpbuf));
   });
   SPH_HIP(hipGetLastError());
@@ -1356,7 +1445,7 @@ int launch_dp_update(sph_ctx *c, int substep, int err_slot, bool accumulate) {
     constexpr int NBv = decltype(D_)::value;                                                \
     LAUNCH_TIMED(c, K_DP_UPDATE, 0,                                                         \
                  (k_dp_update<COL, BAT, NBv><<<nb, bs, 0, c->stream>>>(                      \
-                     n, nb, c->pos, out, c->rho, c->list, c->list_cnt, c->dev, scale, c->sdf, \
+                     n, nb, c->pos, out, c->rho, c->list, c->list_head, c->dev, scale, c->sdf, \
                      c->stats, err_slot, tol, acc, stiff, B)));                             \
   })
   if (c->nscenes) {
@@ -1391,7 +1480,7 @@ int launch_warm_lambda(sph_ctx *c, bool log) {
 #define WARM(LOG, BAT)                                                                      \
   LAUNCH_TIMED(c, K_WARM_LAMBDA, 0,                                                         \
                (k_warm_lambda<LOG, BAT><<<nb, bs, 0, c->stream>>>(                       \
-                   n, nb, c->pos, c->xold, c->list, c->list_cnt, c->dev, c->rho, c->stats,   \
+                   n, nb, c->pos, c->xold, c->list, c->list_head, c->dev, c->rho, c->stats,   \
                    c->id_s, c->lam_prev, c->sdf, LOG ? nullptr : c->warm_net, c->lam_acc,   \
                    LOG ? c->feat : nullptr, LOG ? nullptr : plain, B)))
   if (c->nscenes) {
@@ -1470,7 +1559,7 @@ int launch_color(sph_ctx *c, bool with_velocity) {
       constexpr int NBv = decltype(D_)::value;
       LAUNCH_TIMED(c, K_COLOR, 0,
                    k_color<true, NBv><<<nb, bs, 0, c->stream>>>(n, nb, c->pos, c->xold, c->list,
-                                                               c->list_cnt, c->dev, c->vel,
+                                                               c->list_head, c->dev, c->vel,
                                                                c->aux));
     });
   else
@@ -1478,7 +1567,7 @@ int launch_color(sph_ctx *c, bool with_velocity) {
       constexpr int NBv = decltype(D_)::value;
       LAUNCH_TIMED(c, K_COLOR, 0,
                    k_color<false, NBv><<<nb, bs, 0, c->stream>>>(n, nb, c->pos, c->xold, c->list,
-                                                                c->list_cnt, c->dev, c->vel,
+                                                                c->list_head, c->dev, c->vel,
                                                                 c->aux));
     });
   SPH_HIP(hipGetLastError());
@@ -1494,7 +1583,7 @@ int launch_surface(sph_ctx *c) {
     constexpr int NBv = decltype(D_)::value;
     LAUNCH_TIMED(c, K_SURFACE, 0,
                k_surface<NBv><<<nb, bs, 0, c->stream>>>(n, nb, c->pos, c->aux, c->list,
-                                                       c->list_cnt, c->dev, c->vel));
+                                                       c->list_head, c->dev, c->vel));
   });
   SPH_HIP(hipGetLastError());
   return SPH_OK;
@@ -1510,7 +1599,7 @@ int launch_xsph(sph_ctx *c) {
       constexpr int NBv = decltype(D_)::value;
       LAUNCH_TIMED(c, K_XSPH, 0,
                    k_xsph<NBv><<<nb, bs, 0, c->stream>>>(n, nb, c->pos, c->vel, c->list,
-                                                        c->list_cnt, c->dev, out));
+                                                        c->list_head, c->dev, out));
     });
     SPH_HIP(hipGetLastError());
   }
@@ -1527,7 +1616,7 @@ int launch_vorticity(sph_ctx *c) {
     constexpr int NBv = decltype(D_)::value;
     LAUNCH_TIMED(c, K_VORTICITY, 0,
                k_vorticity<NBv><<<nb, bs, 0, c->stream>>>(n, nb, c->pos, c->vel, c->list,
-                                                         c->list_cnt, c->dev, c->aux, c->pw));
+                                                         c->list_head, c->dev, c->aux, c->pw));
   });
   SPH_HIP(hipGetLastError());
   return SPH_OK;
@@ -1544,7 +1633,7 @@ int launch_confine(sph_ctx *c, bool commit) {
         constexpr int NBv = decltype(D_)::value;
         LAUNCH_TIMED(c, K_CONFINE, 0,
                        k_confine<true, NBv><<<nb, bs, 0, c->stream>>>(
-                           n, nb, c->pw, c->aux, c->list, c->list_cnt, c->dev, c->vel, out,
+                           n, nb, c->pw, c->aux, c->list, c->list_head, c->dev, c->vel, out,
                            c->id_s, c->x_can, c->v_can));
       });
     else
@@ -1552,7 +1641,7 @@ int launch_confine(sph_ctx *c, bool commit) {
         constexpr int NBv = decltype(D_)::value;
         LAUNCH_TIMED(c, K_CONFINE, 0,
                        k_confine<false, NBv><<<nb, bs, 0, c->stream>>>(
-                           n, nb, c->pw, c->aux, c->list, c->list_cnt, c->dev, c->vel, out,
+                           n, nb, c->pw, c->aux, c->list, c->list_head, c->dev, c->vel, out,
                            c->id_s, c->x_can, c->v_can));
       });
     SPH_HIP(hipGetLastError());

@@ -99,3 +99,53 @@ def test_autotune_keeps_state_and_results():
         ref.step(1)
         tuned.step(1)
         _same(ref, tuned, step)
+
+
+def test_list_format_does_not_change_results():
+    """Compact and wide neighbour lists give bit-identical substeps (same
+    neighbours in the same order), eager and graph-replayed."""
+    sc, (compact, wide) = _pair(1e-3)
+    wide.set_tuning(wide_lists=True)
+    assert wide.tuning()["wide_lists"] and not compact.tuning()["wide_lists"]
+    for step in range(4):
+        compact.step(1)
+        wide.step(1)
+        _same(compact, wide, step)
+    ia, ca = compact.neighbors()
+    ib, cb = wide.neighbors()
+    assert np.array_equal(ca, cb) and np.array_equal(ia, ib)
+
+
+def test_set_tuning_validates_and_round_trips():
+    sc, (a, b) = _pair(0.0)
+    a.set_tuning(block={"density_lambda": 64, "xsph": 128}, depth={"dp_update": 32},
+                 split_collide=False, wide_lists=True)
+    t = a.tuning()
+    assert t["block"]["density_lambda"] == 64 and t["block"]["xsph"] == 128
+    assert t["depth"]["dp_update"] == 32 and t["depth"]["density_lambda"] == 16
+    assert t["wide_lists"] and not t["split_collide"]
+    with pytest.raises(sa.SphError):
+        a.set_tuning(block={"color": 96})
+    with pytest.raises(sa.SphError):
+        a.set_tuning(depth={"color": 12})
+    for step in range(3):
+        a.step(1)
+        b.step(1)
+        _same(a, b, step)
+
+
+@pytest.mark.parametrize("depth,block,wide", [(8, 64, False), (32, 128, False), (8, 128, True),
+                                              (32, 64, True)])
+def test_launch_choices_do_not_change_results(depth, block, wide):
+    """Every gather kernel at another depth (its own template instance) and
+    block size, compact or wide lists: bit-identical to the defaults."""
+    sc, (ref, other) = _pair(1e-3)
+    names = [k for k in sa.kernel_names()
+             if k in ("build_lists", "density_lambda", "dp_update", "color", "surface", "xsph",
+                      "vorticity", "confine")]
+    other.set_tuning(block={k: block for k in names}, depth={k: depth for k in names},
+                     wide_lists=wide, split_collide=False)
+    for step in range(3):
+        ref.step(1)
+        other.step(1)
+        _same(ref, other, step)

@@ -77,8 +77,9 @@ def _scene_cfg0(n=(32, 32, 32), spread=1.0, seed=0, **kw):
     return sc, pos, vel
 
 
-def _staged_search(sc, pos, vel):
+def _staged_search(sc, pos, vel, wide=False):
     ctx = sc.context(len(pos))
+    ctx.set_tuning(wide_lists=wide)
     ctx.upload(pos, vel)
     ctx.apply_gravity()
     ctx.predict_positions()
@@ -86,10 +87,12 @@ def _staged_search(sc, pos, vel):
     return ctx
 
 
+@pytest.mark.parametrize("wide", [False, True])
 @pytest.mark.parametrize("spread,seed", [(1.0, 0), (0.55, 1)])
-def test_neighbor_search_bit_exact(spread, seed):
+def test_neighbor_search_bit_exact(spread, seed, wide):
+    """Compact (16-bit offsets) and wide (32-bit) list formats alike."""
     sc, pos, vel = _scene_cfg0(spread=spread, seed=seed)
-    ctx = _staged_search(sc, pos, vel)
+    ctx = _staged_search(sc, pos, vel, wide)
     n = len(pos)
     # x* against the oracle's ApplyGravity + PredictPositions from the same x, v
     x, y, z = _soa(pos)
@@ -118,6 +121,35 @@ def test_neighbor_search_bit_exact(spread, seed):
     assert st_gpu.cell_clamps == 0
     if spread < 1.0:
         assert cnt.max() == orc.MAXN and ol["dropped"] > 0
+
+
+def test_neighbor_search_long_layers_bit_exact():
+    """A bar 8400 particles long and 3 cells thick: a dz layer of the middle
+    cell row spans more than 2^16 sorted indices, so those wavefronts (and
+    those straddling rows) store 32-bit lists while the others stay compact."""
+    s = 0.004
+    sc = Scene(s, (34.0, 0.05, 0.05))
+    pos, vel = block_scene((8400, 6, 6), s, (0.0, 0.0, 0.0), jitter=0.01, seed=5)
+    ctx = _staged_search(sc, pos, vel)
+    n = len(pos)
+    xs, ys, zs = _soa(ctx.field(sa.FIELD_POSITION_STAR))
+    ol = oracle_lists(sc.o, xs, ys, zs)
+    # precondition: some particle's layer spans more than 2^16 indices
+    o = sc.o
+    key = ol["keys"].astype(np.int64)
+    cz, rem = key // (o.nx * o.ny), key % (o.nx * o.ny)
+    cy, cx = rem // o.nx, rem % o.nx
+    y0, y1 = np.maximum(cy - 1, 0), np.minimum(cy + 1, o.ny - 1)
+    x0, x1 = np.maximum(cx - 1, 0), np.minimum(cx + 1, o.nx - 1)
+    row0 = cz * o.nx * o.ny + y0 * o.nx + x0
+    row1 = cz * o.nx * o.ny + y1 * o.nx + x1
+    span = ol["end"][row1].astype(np.int64) - ol["start"][row0].astype(np.int64)
+    assert span.max() > 2 ** 16 and span.min() < 2 ** 16
+    ids, cnt = ctx.neighbors()
+    assert np.array_equal(cnt, ol["cnt"])
+    mask = np.arange(orc.MAXN)[None, :] < cnt[:, None]
+    assert np.array_equal(ids.reshape(n, -1)[mask], ol["nbr"].reshape(n, -1)[mask])
+    record("long_layers", particles=n, max_span=int(span.max()))
 
 
 # ---------------------------------------------------------------------------
