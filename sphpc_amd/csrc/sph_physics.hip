@@ -103,21 +103,29 @@ __device__ __forceinline__ void load_compact(const NbList &nl, int k0, uint32_t 
   static_assert(NB % 8 == 0, "compact quads hold 8 entries");
   const uint4 *Q = reinterpret_cast<const uint4 *>(nl.L);
   const int q0 = k0 >> 3;
+  // values, not member lvalues: a select between member addresses would
+  // keep the struct in memory (and the compiler moves such memory to LDS)
+  const uint32_t b0 = nl.h.x, b1 = nl.h.y, b2 = nl.h.z;
+  const int e0 = nl.e0, e1 = nl.e1, c = nl.c;
 #pragma unroll
   for (int u = 0; u < NB / 8; ++u) {
     uint4 a = make_uint4(0u, 0u, 0u, 0u);
-    if (k0 + 8 * u < nl.c) a = Q[(size_t)(q0 + u) * kWave];
+    if (k0 + 8 * u < c) a = Q[(size_t)(q0 + u) * kWave];
     const uint32_t w[4] = {a.x, a.y, a.z, a.w};
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
       const int k = k0 + 8 * u + t;
       const uint32_t off = (t & 1) ? (w[t >> 1] >> 16) : (w[t >> 1] & 0xFFFFu);
-      const uint32_t b = k < nl.e0 ? nl.h.x : (k < nl.e1 ? nl.h.y : nl.h.z);
-      j[8 * u + t] = b + off;
+      const uint32_t lb = k < e1 ? b1 : b2;
+      j[8 * u + t] = (k < e0 ? b0 : lb) + off;
     }
   }
 }
 
+// A wavefront runs each batch for its longest list.  Batches in which no lane's
+// list ends skip the per-neighbour count test, and a last batch in which every
+// lane ends within NB/2 neighbours runs only NB/2 of them (the wave's longest
+// list, not the mean, sets the work: half batches cut the padding slots).
 template <typename T, int NB, bool PREFETCH, bool WIDE, typename Load, typename Body>
 __device__ __forceinline__ void gather_loop(const NbList &nl, uint32_t self, Load &load,
                                             Body &body) {
@@ -128,16 +136,32 @@ __device__ __forceinline__ void gather_loop(const NbList &nl, uint32_t self, Loa
     else
       load_compact<NB>(nl, k0, jn);
   };
+  const int c = nl.c;
   fetch(0);
-  for (int k0 = 0; k0 < nl.c; k0 += NB) {
+  for (int k0 = 0; k0 < c; k0 += NB) {
+    // the lanes still in the loop have c > k0
     if (!PREFETCH && k0 > 0) fetch(k0);
-    T q[NB];
+    if (NB >= 16 && __ballot(k0 + NB / 2 < c) == 0ull) {
+      T q[NB / 2];
 #pragma unroll
-    for (int b = 0; b < NB; ++b) q[b] = load(jn[b]);
-    if (PREFETCH) fetch(k0 + NB);
+      for (int b = 0; b < NB / 2; ++b) q[b] = load(jn[b]);
 #pragma unroll
-    for (int b = 0; b < NB; ++b)
-      if (k0 + b < nl.c) body(q[b]);
+      for (int b = 0; b < NB / 2; ++b)
+        if (k0 + b < c) body(q[b]);
+    } else {
+      T q[NB];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) q[b] = load(jn[b]);
+      if (PREFETCH) fetch(k0 + NB);
+      if (__ballot(k0 + NB > c) == 0ull) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b) body(q[b]);
+      } else {
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+          if (k0 + b < c) body(q[b]);
+      }
+    }
   }
 }
 
@@ -562,7 +586,8 @@ __global__ __launch_bounds__(kBlock) void k_density_lambda(
         const float r2c = fmaxf(r2, P.rmin2);
         const float ir = fast_rsq(r2c);
         const float hr = fmaf(-r2c, ir, P.h);  // h - r, r floored at 1e-4 h
-        const float g = hr > 0.0f ? hr * hr * ir : 0.0f;  // DelW(r)/r / norm
+        const float hp = fmaxf(hr, 0.0f);
+        const float g = hp * hp * ir;  // DelW(r)/r / norm
         sgx = fmaf(g, dx, sgx);
         sgy = fmaf(g, dy, sgy);
         sgz = fmaf(g, dz, sgz);
@@ -646,7 +671,8 @@ __device__ __forceinline__ float3 dp_sum(const float4 &p, const float4 *pos, con
     const float r2c = fmaxf(sq3(dx, dy, dz), P.rmin2);
     const float ir = fast_rsq(r2c);
     const float hr = fmaf(-r2c, ir, P.h);
-    const float g = hr > 0.0f ? hr * hr * ir : 0.0f;
+    const float hp = fmaxf(hr, 0.0f);
+    const float g = hp * hp * ir;
     const float f = (p.w + q.w) * g;
     ax = fmaf(f, dx, ax);
     ay = fmaf(f, dy, ay);
@@ -793,7 +819,8 @@ __global__ __launch_bounds__(kBlock) void k_warm_lambda(
       const float r2c = fmaxf(r2, P.rmin2);
       const float ir = fast_rsq(r2c);
       const float hr = fmaf(-r2c, ir, P.h);
-      const float g = hr > 0.0f ? hr * hr * ir : 0.0f;
+      const float hp = fmaxf(hr, 0.0f);
+    const float g = hp * hp * ir;
       sgx = fmaf(g, dx, sgx);
       sgy = fmaf(g, dy, sgy);
       sgz = fmaf(g, dz, sgz);
@@ -942,7 +969,8 @@ __global__ __launch_bounds__(kBlock) void k_color(uint32_t n, uint32_t nblocks,
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     const float r2c = fmaxf(sq3(dx, dy, dz), P.rmin2);
     const float u = P.h2 - r2c;
-    const float g = u > 0.0f ? u * u * fast_rcp(q.w) : 0.0f;
+    const float up = fmaxf(u, 0.0f);
+    const float g = up * up * fast_rcp(q.w);
     ax = fmaf(g, dx, ax);
     ay = fmaf(g, dy, ay);
     az = fmaf(g, dz, az);
@@ -1013,7 +1041,8 @@ __global__ __launch_bounds__(kBlock) void k_xsph(uint32_t n, uint32_t nblocks,
     const float4 &q = qv.a, &vq = qv.b;
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     const float u = P.h2 - fmaf(dy * dy * dz, dz, dx * dx);
-    const float w = u > 0.0f ? u * u * u * fast_rcp(q.w) : 0.0f;
+    const float up = fmaxf(u, 0.0f);
+    const float w = up * up * up * fast_rcp(q.w);
     ax = fmaf(vq.x - vi.x, w, ax);
     ay = fmaf(vq.y - vi.y, w, ay);
     az = fmaf(vq.z - vi.z, w, az);
@@ -1045,7 +1074,8 @@ __global__ __launch_bounds__(kBlock) void k_vorticity(uint32_t n, uint32_t nbloc
     const float r2c = fmaxf(sq3(dx, dy, dz), P.rmin2);
     const float ir = fast_rsq(r2c);
     const float hr = fmaf(-r2c, ir, P.h);
-    float g = hr > 0.0f ? hr * hr * ir : 0.0f;
+    const float hp = fmaxf(hr, 0.0f);
+    float g = hp * hp * ir;
     if (P.vort_model == 1) g *= fast_rcp(q.w);
     const float wx = g * dx, wy = g * dy, wz = g * dz;
     const float vx = vq.x - vi.x, vy = vq.y - vi.y, vz = vq.z - vi.z;
@@ -1084,7 +1114,8 @@ __global__ __launch_bounds__(kBlock) void k_confine(
     const float r2c = fmaxf(sq3(dx, dy, dz), P.rmin2);
     const float ir = fast_rsq(r2c);
     const float hr = fmaf(-r2c, ir, P.h);
-    const float g = hr > 0.0f ? q.w * hr * hr * ir : 0.0f;
+    const float hp = fmaxf(hr, 0.0f);
+    const float g = q.w * hp * hp * ir;
     ex = fmaf(g, dx, ex);
     ey = fmaf(g, dy, ey);
     ez = fmaf(g, dz, ez);
