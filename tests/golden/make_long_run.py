@@ -1,0 +1,85 @@
+"""Writes long_run.json: statistics of long FP64 oracle runs (the oracle's
+stages are pinned bit-for-bit to the reference's own C by
+tests/test_oracle_vs_ref.py; the substep composes them in the order of
+Simulation/src/simulation.c:54-116).  Trajectories of a dam break diverge
+chaotically between FP32 and FP64, so the GPU is compared with these by
+statistics: maximum, 99th percentile and mean compression error of the last
+density pass,
+kinetic energy, mean neighbour count, particles at the 60-neighbour cap.
+
+Run from the repository root: python3 tests/golden/make_long_run.py"""
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+import orc  # noqa: E402
+
+CASES = {
+    # configs[0] of the benchmark: plain PBF, one 32^3 block in a corner
+    "plain_dam_break_32k": dict(
+        lattice=dict(dims=(32, 32, 32), s=0.02, origin=(0.0, 0.0, 0.0), jitter=0.01, seed=2024),
+        box=(2.0, 1.0, 1.0), iters=3, substeps=500, every=50,
+        physics=dict(lambda_eps=600.0)),
+    # the 8M double dam break at a quarter of its lattice per axis, all terms and the
+    # torus + box obstacle
+    "double_dam_break_131k": dict(
+        workload="double_dam_break", scale=0.25, iters=4, substeps=200, every=20,
+        physics=dict(k=0.5, c=0.01, lambda_eps=600.0, vort_eps=0.01, vort_model=1,
+                     surface=True, viscosity=True, vorticity=True)),
+}
+
+
+def scene(case):
+    """Initial FP32 positions, box, obstacle (or None) of a case."""
+    if "lattice" in case:
+        lt = case["lattice"]
+        pos = orc.lattice_block(*lt["dims"], lt["s"], origin=lt["origin"], jitter=lt["jitter"],
+                                seed=lt["seed"])
+        return lt["s"], pos, case["box"], None
+    from sphpc_amd import workloads
+    sc = workloads.double_dam_break(case["scale"])
+    return sc["s"], sc["pos"], sc["box"], sc
+
+
+def statistics(rho, v, cnt, s, rest=1000.0):
+    e = np.maximum(rho - rest, 0.0) / rest
+    ke = 0.5 * rest * s ** 3 * float(np.sum(v.astype(np.float64) ** 2))
+    return dict(max_err=float(e.max()), p99_err=float(np.quantile(e, 0.99)),
+                mean_err=float(e.mean()), ke=ke, mean_nb=float(cnt.mean()),
+                at_cap=int(np.sum(cnt >= orc.MAXN)))
+
+
+def run(name, case):
+    s, pos, box, sc = scene(case)
+    L = orc.oracle()
+    p = orc.make_params(s, box, dt=1.0 / 300.0, g=9.81, **case["physics"])
+    ob = orc.obstacle(sc["sdf"], sc["dims"], sc["dx"], sc["wmin"], sc["wmax"]) if sc else None
+    n = len(pos)
+    cols = [np.ascontiguousarray(pos[:, i], dtype=np.float64) for i in range(3)]
+    a = dict(x=cols[0], y=cols[1], z=cols[2], vx=np.zeros(n), vy=np.zeros(n), vz=np.zeros(n),
+             xs=np.zeros(n), ys=np.zeros(n), zs=np.zeros(n), density=np.zeros(n),
+             lambda_=np.zeros(n), nbr=np.zeros(n * orc.MAXN, np.uint32), cnt=np.zeros(n, np.int32))
+    st = orc.OrcState(**{k: v.ctypes.data for k, v in a.items()})
+    out = []
+    t0 = time.time()
+    for i in range(case["substeps"]):
+        L.orc_substep(n, C.byref(st), C.byref(p), C.byref(ob) if ob else None, case["iters"],
+                      0.0, None)
+        if (i + 1) % case["every"] == 0:
+            v = np.stack([a["vx"], a["vy"], a["vz"]], 1)
+            out.append(dict(substep=i + 1, **statistics(a["density"], v, a["cnt"], s)))
+            print(name, out[-1], f"{time.time() - t0:.0f}s", flush=True)
+    return dict(particles=n, **{k: v for k, v in case.items()}, checkpoints=out)
+
+
+if __name__ == "__main__":
+    res = {name: run(name, case) for name, case in CASES.items()}
+    with open(os.path.join(HERE, "long_run.json"), "w") as f:
+        json.dump(res, f, indent=1)
