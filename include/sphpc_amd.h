@@ -188,7 +188,7 @@ enum sph_field {
 };
 int sph_download_field(sph_ctx *ctx, int field, float *out);
 /* Neighbour lists as particle ids in the reference order (NeighborBucket,
- * neighbors.h:25-28): ids[i*60 + j], j < counts[i]. */
+ * neighbors.h:25-28): ids[i*60 + j], j < counts[i].  ids NULL: counts only. */
 int sph_download_neighbors(sph_ctx *ctx, uint32_t *ids, int *counts);
 /* Sorted (cell key, particle id) pairs of the last neighbour search. */
 int sph_download_sorted(sph_ctx *ctx, uint32_t *keys, uint32_t *ids);

@@ -381,6 +381,13 @@ class Context:
                                            cnt.ctypes.data_as(C.POINTER(C.c_int))))
         return ids, cnt
 
+    def neighbor_counts(self):
+        """Neighbour count per particle id (the lists themselves not downloaded)."""
+        cnt = np.zeros(max(self.count(), 1), np.int32)
+        check(lib().sph_download_neighbors(self._h, None,
+                                           cnt.ctypes.data_as(C.POINTER(C.c_int))))
+        return cnt
+
     def sorted_pairs(self):
         n = self.count()
         k = np.zeros(max(n, 1), np.uint32)

@@ -1002,12 +1002,16 @@ extern "C" int sph_download_field(sph_ctx *c, int field, float *out) {
 extern "C" int sph_download_neighbors(sph_ctx *c, uint32_t *out_ids, int *counts) {
   int rc = require_lists(c);
   if (rc) return rc;
-  if (!out_ids || !counts) return arg_error("NULL argument");
+  if (!counts) return arg_error("NULL argument");
   const int n = c->n;
   const size_t groups = ((size_t)n + kWave - 1) / kWave;
   std::vector<uint32_t> ids, list, head;
   if ((rc = fetch(c, c->id_s, (size_t)n, &ids))) return rc;
   if ((rc = fetch(c, (const uint32_t *)c->list_head, (size_t)n * 4, &head))) return rc;
+  if (!out_ids) {
+    for (int s = 0; s < n; ++s) counts[ids[s]] = (int)(head[(size_t)s * 4 + 3] & kHeadCount);
+    return SPH_OK;
+  }
   if ((rc = fetch(c, (const uint32_t *)c->list, groups * kListStride, &list))) return rc;
   for (int s = 0; s < n; ++s) {
     const uint32_t id = ids[s];

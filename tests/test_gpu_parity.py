@@ -114,6 +114,7 @@ def test_neighbor_search_bit_exact(spread, seed, wide):
     assert np.array_equal(st, ol["start"]) and np.array_equal(en, ol["end"])
     ids, cnt = ctx.neighbors()
     assert np.array_equal(cnt, ol["cnt"])
+    assert np.array_equal(ctx.neighbor_counts(), cnt)
     mask = np.arange(orc.MAXN)[None, :] < cnt[:, None]
     assert np.array_equal(ids.reshape(n, -1)[mask], ol["nbr"].reshape(n, -1)[mask])
     st_gpu = ctx.stats()
