@@ -68,8 +68,10 @@ def read_traffic(kernel, workload_tag):
         return None
     for line in open(path):
         parts = line.split()
-        if len(parts) >= 2 and parts[0] == kernel and parts[1].startswith("bytes_per_launch="):
-            return float(parts[1].split("=", 1)[1])
+        if parts and parts[0] == kernel:
+            for tok in parts[1:]:
+                if tok.startswith("bytes_per_launch="):
+                    return float(tok.split("=", 1)[1])
     return None
 
 

@@ -93,7 +93,9 @@ typedef struct sph_stats {
   unsigned long long cell_clamps;       /* cell coordinates clamped into the grid */
   long long substeps;        /* substeps run on this context */
   unsigned long long neighbor_pairs;    /* sum of list lengths of the last search */
-  int warm_fallback;         /* warm start reverted to the plain solve (last substep) */
+  int warm_fallback;         /* warm start reverted to the plain solve in the last substep:
+                                1, or in a batch the number of scenes reverted (each
+                                scene decides on its own maximum error) */
 } sph_stats;
 
 typedef struct sph_ctx sph_ctx;

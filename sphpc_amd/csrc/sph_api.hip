@@ -175,7 +175,7 @@ static void free_all(sph_ctx *c) {
                   c->vel_a, c->vel_b, c->aux, c->dpbuf, c->rho, c->list, c->list_head,
                   c->stats, c->pair_partial, c->red, c->sdf_buf, c->scan.tile_sums,
                   c->pw, c->lam_acc, c->lam_prev, c->feat, c->scenes_dev, c->scene_of,
-                  c->batch_sdf, c->scene_err,
+                  c->batch_sdf, c->scene_err, c->scene_err0,
                   c->warm_net};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
@@ -414,6 +414,7 @@ extern "C" int sph_create_batch(const sph_params *params, int nscenes, const sph
   hipError_t e = hipMalloc((void **)&c->scenes_dev, sizeof(SceneDev) * nscenes);
   if (e == hipSuccess) e = hipMalloc((void **)&c->scene_of, sizeof(uint16_t) * np);
   if (e == hipSuccess) e = hipMalloc((void **)&c->scene_err, sizeof(uint32_t) * nscenes);
+  if (e == hipSuccess) e = hipMalloc((void **)&c->scene_err0, sizeof(uint32_t) * nscenes);
   if (e == hipSuccess && sdf_words) e = hipMalloc((void **)&c->batch_sdf, sizeof(float) * sdf_words);
   size_t off = 0;
   for (int k = 0; k < nscenes && e == hipSuccess; ++k) {
@@ -713,15 +714,21 @@ static int substep_launch(sph_ctx *c, bool *warm_out) {
   *warm_out = warm;
   int it0 = 0;
   const float4 *pre_warm = nullptr;
+  const bool fallback = warm && !c->log_on && iters > 1;
   if (warm) {
     if ((rc = launch_warm_lambda(c, c->log_on))) return rc;
+    // a batch decides the fallback per scene: errors before the step
+    if (fallback && c->nscenes && (rc = launch_scene_errors(c, c->scene_err0))) return rc;
     pre_warm = c->pos;
     if ((rc = launch_dp_update(c, 0, 0, true))) return rc;
     it0 = 1;
   }
   for (int it = it0; it < iters; ++it) {
     if ((rc = launch_density(c, 0, it))) return rc;
-    if (it == 1 && warm && !c->log_on && (rc = launch_warm_fallback(c, pre_warm))) return rc;
+    if (it == 1 && fallback) {
+      if (c->nscenes && (rc = launch_scene_errors(c))) return rc;
+      if ((rc = launch_warm_fallback(c, pre_warm))) return rc;
+    }
     if ((rc = launch_dp_update(c, it, it, warm))) return rc;
   }
   c->last_err_slot = iters > 0 ? iters - 1 : -1;

@@ -190,6 +190,7 @@ struct sph_ctx {
   uint16_t *scene_of = nullptr;
   float *batch_sdf = nullptr;
   uint32_t *scene_err = nullptr;
+  uint32_t *scene_err0 = nullptr;  // per scene: error before a warm-started step
   std::vector<uint32_t> scene_first;  // nscenes + 1 particle offsets
   sph::BatchDev batch() const { return sph::BatchDev{scenes_dev, scene_of}; }
   // launch choices (sph_autotune)
@@ -270,7 +271,7 @@ int launch_warm_lambda(sph_ctx *c, bool log);
 int launch_scatter_lambda(sph_ctx *c);
 int launch_warm_samples(sph_ctx *c, float *x, float *t, uint32_t count, int every);
 int launch_warm_fallback(sph_ctx *c, const float4 *pre_warm);
-int launch_scene_errors(sph_ctx *c);
+int launch_scene_errors(sph_ctx *c, uint32_t *dst = nullptr);
 int mlp_pack(sph_mlp *m, MlpPacked *dst, hipStream_t s);  // sph_mlp.hip
 int configure_kernels();  // LDS limits of the tiled kernels, once per process
 

@@ -897,6 +897,55 @@ __global__ __launch_bounds__(kBlock) void k_warm_fallback(uint32_t n, uint32_t *
   }
 }
 
+// The same for a batch of scenes, scene by scene: a scene whose maximum error
+// after the step (e1) exceeds the one before it (e0) is reverted.  A second
+// kernel sets the iteration-1 error slot to the maximum over scenes of the
+// error each scene continues from, and counts the reverted scenes.
+__global__ __launch_bounds__(kBlock) void k_warm_fallback_scenes(
+    uint32_t n, const uint32_t *stats, const uint16_t *__restrict__ scene_of,
+    const uint32_t *__restrict__ e0, const uint32_t *__restrict__ e1,
+    const float4 *__restrict__ pre, float4 *__restrict__ pos,
+    const float2 *__restrict__ plain, float *__restrict__ rho, float *__restrict__ lam_acc) {
+  if (stats[STAT_CONVERGED]) return;
+  const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= n) return;
+  const uint32_t k = scene_of[s];
+  if (e1[k] <= e0[k]) return;  // non-negative floats compare as their bit patterns
+  const float4 p = pre[s];
+  const float2 q = plain[s];
+  pos[s] = make_float4(p.x, p.y, p.z, q.x);
+  rho[s] = q.y;
+  lam_acc[s] = 0.0f;
+}
+
+__global__ __launch_bounds__(kBlock) void k_warm_fallback_stats(uint32_t nscenes,
+                                                                uint32_t *stats,
+                                                                const uint32_t *__restrict__ e0,
+                                                                const uint32_t *__restrict__ e1) {
+  if (stats[STAT_CONVERGED]) return;
+  uint32_t m = 0u, reverted = 0u;
+  for (uint32_t k = threadIdx.x; k < nscenes; k += blockDim.x) {
+    const bool back = e1[k] > e0[k];
+    m = max(m, back ? e0[k] : e1[k]);
+    reverted += back ? 1u : 0u;
+  }
+  __shared__ uint32_t sm[kBlock], sr[kBlock];
+  sm[threadIdx.x] = m;
+  sr[threadIdx.x] = reverted;
+  __syncthreads();
+  for (uint32_t o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      sm[threadIdx.x] = max(sm[threadIdx.x], sm[threadIdx.x + o]);
+      sr[threadIdx.x] += sr[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    stats[STAT_ERR0 + 1] = sm[0];
+    stats[STAT_FALLBACK] = sr[0];
+  }
+}
+
 // Keeps each particle's accumulated multiplier in particle-id order.
 __global__ __launch_bounds__(kBlock) void k_scatter_lambda(uint32_t n,
                                                            const uint32_t *__restrict__ id_s,
@@ -1527,6 +1576,18 @@ int launch_warm_lambda(sph_ctx *c, bool log) {
 int launch_warm_fallback(sph_ctx *c, const float4 *pre_warm) {
   const uint32_t n = c->n;
   if (!n) return SPH_OK;
+  if (c->nscenes) {
+    // per scene; scene_err0 / scene_err hold the errors before and after the step
+    LAUNCH_TIMED(c, K_AUX, 0,
+                 k_warm_fallback_scenes<<<blocks_for(n), kBlock, 0, c->stream>>>(
+                     n, c->stats, c->scene_of, c->scene_err0, c->scene_err, pre_warm, c->pos,
+                     reinterpret_cast<const float2 *>(c->dpbuf), c->rho, c->lam_acc));
+    SPH_HIP(hipGetLastError());
+    k_warm_fallback_stats<<<1, kBlock, 0, c->stream>>>((uint32_t)c->nscenes, c->stats,
+                                                       c->scene_err0, c->scene_err);
+    SPH_HIP(hipGetLastError());
+    return SPH_OK;
+  }
   LAUNCH_TIMED(c, K_AUX, 0,
                k_warm_fallback<<<blocks_for(n), kBlock, 0, c->stream>>>(
                    n, c->stats, pre_warm, c->pos, reinterpret_cast<const float2 *>(c->dpbuf),
@@ -1535,14 +1596,15 @@ int launch_warm_fallback(sph_ctx *c, const float4 *pre_warm) {
   return SPH_OK;
 }
 
-int launch_scene_errors(sph_ctx *c) {
+int launch_scene_errors(sph_ctx *c, uint32_t *dst) {
   const uint32_t ns = (uint32_t)c->nscenes;
   if (!ns) return SPH_OK;
-  k_fill_words<<<blocks_for(ns), kBlock, 0, c->stream>>>(ns, c->scene_err, 0u);
+  if (!dst) dst = c->scene_err;
+  k_fill_words<<<blocks_for(ns), kBlock, 0, c->stream>>>(ns, dst, 0u);
   SPH_HIP(hipGetLastError());
   if (c->n) {
     k_scene_errors<<<blocks_for(c->n), kBlock, 0, c->stream>>>(c->n, c->rho, c->scene_of, c->dev,
-                                                                c->scene_err);
+                                                                dst);
     SPH_HIP(hipGetLastError());
   }
   return SPH_OK;

@@ -94,3 +94,51 @@ def test_batch_rejects_partial_upload_and_obstacle_change():
         batch.upload(scenes[0]["pos"])
     with pytest.raises(sa.SphError):
         batch.set_obstacle(np.zeros(8, np.float32), (2, 2, 2), 0.1, (0, 0, 0), (0.2, 0.2, 0.2))
+
+
+def test_batch_warm_start_falls_back_per_scene():
+    """Warm start in a batch with a fixed iteration count: each scene decides
+    the plain fallback on its own maximum error, exactly as a context of its own
+    does, so every scene is bit-identical to a separate warm-started context and
+    the batch reports the number of scenes that fell back.  Compressed blocks
+    (the +x-face prediction raises their error) and sparse ones (they stay
+    below rest density) alternate, so both outcomes occur."""
+    from test_gpu_warm import _face_net
+    s = float(np.float32(0.02))
+    rng = np.random.default_rng(11)
+    scenes = []
+    for k in range(6):
+        f = 0.97 if k % 2 == 0 else 1.3
+        nb = [int(rng.integers(9, 14)), int(rng.integers(8, 12)), int(rng.integers(8, 12))]
+        pos = sa.scene_lattice([sa.Block(*nb, 0.05, 0.02, 0.0)], s * f, jitter=0.01, seed=20 + k)
+        vel = rng.normal(0, 0.05, pos.shape).astype(np.float32)
+        scenes.append(dict(particles=len(pos), size=(0.6, 0.5, 0.5), sdf=None, pos=pos, vel=vel))
+    p = _params(s, iters=4)
+    net = _face_net(40.0)
+    batch = sa.BatchContext(p, scenes)
+    batch.upload(np.concatenate([sc["pos"] for sc in scenes]),
+                 np.concatenate([sc["vel"] for sc in scenes]))
+    batch.set_warm_start(net)
+    singles = []
+    for sc in scenes:
+        c = sa.Context(p, sc["size"], sc["particles"])
+        c.upload(sc["pos"], sc["vel"])
+        c.set_warm_start(net)
+        singles.append(c)
+    for step in range(3):
+        batch.step(1)
+        falls = 0
+        for c in singles:
+            c.step(1)
+            falls += c.stats().warm_fallback
+        bp, bv = batch.download()
+        for k, (sc, c) in enumerate(zip(scenes, singles)):
+            a, b = batch.first[k], batch.first[k + 1]
+            sp, sv = c.download()
+            assert np.array_equal(bp[a:b], sp), (step, k)
+            assert np.array_equal(bv[a:b], sv), (step, k)
+        assert batch.stats().warm_fallback == falls, step
+        if step == 0:
+            first = falls
+            assert 0 < falls < len(scenes), falls  # both outcomes occur
+    record("batch_warm_fallback", scenes=len(scenes), first_substep_fallbacks=first)
