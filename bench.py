@@ -47,7 +47,7 @@ KERNEL_BYTES = {
 }
 
 
-def scene_cfg2(scale=1.0):
+def scene_double_dam_break(scale=1.0):
     from sphpc_amd import workloads
     return workloads.double_dam_break(scale)
 
@@ -185,7 +185,7 @@ def bench_batch(args):
     nbar = st.neighbor_pairs / max(n, 1)
     prof_steps = max(2, args.steps // 5)
     _, times = timed_region(ctx, prof_steps)
-    roof, per = roofline(times, n, nbar, prof_steps, f"batch{args.scenes}")
+    roof, per = roofline(times, n, nbar, prof_steps, f"pbf_substep_batch{args.scenes}")
     counts = [sc["particles"] for sc in scenes]
     out = {
         "metric": f"particle-updates/s, {args.scenes} batched scenes",
@@ -254,7 +254,7 @@ def main():
         bench_batch(args)
         return
 
-    sc = scene_cfg2(args.scale)
+    sc = scene_double_dam_break(args.scale)
     n = len(sc["pos"])
     ctx = sa.Context(gpu_params(sc), sc["box"], n)
     ctx.upload(sc["pos"])
@@ -298,7 +298,7 @@ def main():
                    "neighbor_overflow": st.neighbor_overflow},
     }
     out["roofline"], out["kernels_ms_per_substep"] = roofline(
-        times, n, nbar, prof_steps, "cfg2" if args.scale == 1.0 else f"scale{args.scale}")
+        times, n, nbar, prof_steps, "pbf_substep_8M" if args.scale == 1.0 else f"pbf_substep_scale{args.scale}")
     out["config"]["graph"] = not args.no_graph
     out["config"]["autotune"] = compact_tuning(tuning)
     if not args.no_extras:

@@ -22,7 +22,7 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 import orc  # noqa: E402
 
 CASES = {
-    # configs[0] of the benchmark: plain PBF, one 32^3 block in a corner
+    # plain PBF dam break: one 32^3 block in the corner of a 2 x 1 x 1 m box
     "plain_dam_break_32k": dict(
         lattice=dict(dims=(32, 32, 32), s=0.02, origin=(0.0, 0.0, 0.0), jitter=0.01, seed=2024),
         box=(2.0, 1.0, 1.0), iters=3, substeps=500, every=50,

@@ -69,7 +69,7 @@ def test_cell_bounds_bit_exact(n, cells):
 # ---------------------------------------------------------------------------
 # Neighbour search in the staged pipeline (neighbors.c:68-298)
 # ---------------------------------------------------------------------------
-def _scene_cfg0(n=(32, 32, 32), spread=1.0, seed=0, **kw):
+def _scene_dam_block(n=(32, 32, 32), spread=1.0, seed=0, **kw):
     s = 0.02
     sc = Scene(s, (2.0, 1.0, 1.0), **kw)
     pos, vel = block_scene(n, s * spread, (0.0, 0.0, 0.0), jitter=0.01, seed=seed,
@@ -91,7 +91,7 @@ def _staged_search(sc, pos, vel, wide=False):
 @pytest.mark.parametrize("spread,seed", [(1.0, 0), (0.55, 1)])
 def test_neighbor_search_bit_exact(spread, seed, wide):
     """Compact (16-bit offsets) and wide (32-bit) list formats alike."""
-    sc, pos, vel = _scene_cfg0(spread=spread, seed=seed)
+    sc, pos, vel = _scene_dam_block(spread=spread, seed=seed)
     ctx = _staged_search(sc, pos, vel, wide)
     n = len(pos)
     # x* against the oracle's ApplyGravity + PredictPositions from the same x, v
@@ -166,7 +166,7 @@ def _sphere(dim=64, center=(0.3, 0.3, 0.1), R=0.15):
 
 
 def test_jacobi_stages_match_oracle():
-    sc, pos, vel = _scene_cfg0(n=(24, 24, 20), seed=3, lambda_eps=600.0, k=0.5, c=0.01,
+    sc, pos, vel = _scene_dam_block(n=(24, 24, 20), seed=3, lambda_eps=600.0, k=0.5, c=0.01,
                                vort_eps=0.01)
     sdf, dims, dx, wmin, wmax = _sphere()
     o_obs = orc.obstacle(sdf, dims, dx, wmin, wmax)
@@ -224,7 +224,7 @@ def test_jacobi_stages_match_oracle():
 
 
 def test_velocity_passes_match_oracle():
-    sc, pos, vel = _scene_cfg0(n=(24, 24, 20), seed=4, lambda_eps=600.0, k=0.5, c=0.01,
+    sc, pos, vel = _scene_dam_block(n=(24, 24, 20), seed=4, lambda_eps=600.0, k=0.5, c=0.01,
                                vort_eps=0.01, vorticity=True)
     ctx = _staged_search(sc, pos, vel)
     n = len(pos)
@@ -304,7 +304,7 @@ def _oracle_state(pos, vel, n):
 
 @pytest.mark.parametrize("fused", [True, False])
 def test_substep_matches_oracle(fused):
-    sc, pos, vel = _scene_cfg0(n=(24, 24, 20), seed=5, lambda_eps=600.0, k=0.5, c=0.01,
+    sc, pos, vel = _scene_dam_block(n=(24, 24, 20), seed=5, lambda_eps=600.0, k=0.5, c=0.01,
                                vort_eps=0.01, iters=4)
     sdf, dims, dx, wmin, wmax = _sphere(center=(0.25, 0.25, 0.05))
     o_obs = orc.obstacle(sdf, dims, dx, wmin, wmax)
@@ -338,7 +338,7 @@ def test_substep_matches_oracle(fused):
 def test_fused_equals_staged_closely():
     # vorticity model 1: the reference formula drives |v| to ~1e3 m/s at this
     # scale in one substep, which only magnifies FP32 ordering differences
-    sc, pos, vel = _scene_cfg0(n=(20, 20, 16), seed=6, lambda_eps=600.0, k=0.5, c=0.01,
+    sc, pos, vel = _scene_dam_block(n=(20, 20, 16), seed=6, lambda_eps=600.0, k=0.5, c=0.01,
                                vort_eps=0.01, iters=3, vort_model=1)
     n = len(pos)
     out = []
@@ -360,8 +360,8 @@ def test_fused_equals_staged_closely():
 
 
 def test_tolerance_mode_iterations():
-    # Plain Jacobi until max density error <= 0.1% (cap 10), configs[3] rule
-    sc, pos, vel = _scene_cfg0(n=(24, 24, 24), seed=7, lambda_eps=600.0, iters=10, tol=1e-3)
+    # Plain Jacobi until max density error <= 0.1% (cap 10), the warm-start workload's rule
+    sc, pos, vel = _scene_dam_block(n=(24, 24, 24), seed=7, lambda_eps=600.0, iters=10, tol=1e-3)
     n = len(pos)
     ctx = sc.context(n)
     ctx.upload(pos, vel)
