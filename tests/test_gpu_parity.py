@@ -4,7 +4,7 @@ Integer/index work (cell keys, sort order, cell tables, neighbour lists with
 the 60 cap) must be bit-exact.  Floating-point stages are compared from the
 same state (the oracle reads the GPU's FP32 inputs promoted to double); the
 tolerance of each comparison is written next to it, and every measured error
-is appended to gpurun_out/parity_errors.jsonl."""
+is appended to the error log of gpu_common (ERR_LOG)."""
 import ctypes as C
 
 import numpy as np
@@ -183,7 +183,7 @@ def test_jacobi_stages_match_oracle():
         rho64 = np.zeros(n)
         L.orc_compute_densities(n, xs, ys, zs, nbr, cnt, C.byref(sc.o), rho64)
         e_rho = float(np.max(np.abs(rho_gpu - rho64) / rho64))
-        # tolerance: north_star's 1e-5 relative, per particle
+        # tolerance: 1e-5 relative, per particle
         assert e_rho <= 1e-5, e_rho
 
         ctx.compute_lambda()

@@ -56,7 +56,7 @@ def test_lattice_lambda_closed_form():
                 for k in _lattice_offsets(4))
     rho = 0.991 * 1000.0 * 315.0 * 330.0 / (64.0 * math.pi * 512.0)
     sum_c = m * m * grad2 / 1000.0 ** 2
-    # Survey's figure: sum |grad C|^2 = 0.36670/s^2
+    # perfect interior lattice: sum |grad C|^2 = 0.36670 / s^2
     assert abs(sum_c * s * s - 0.36670) < 5e-5
     lam_expected = -(rho / 1000.0 - 1.0) / (sum_c + 0.01)
     pos = orc.lattice_block(12, 12, 12, s, origin=(0.2, 0.2, 0.2)).astype(np.float64)
