@@ -5,6 +5,11 @@ scale 1) at s = 0.008 m, seeded +-0.01 s jitter, at the two x-ends of a
 6 x 2 x 2 m box, with a torus (R 0.4, r 0.12, axis z) united with an
 axis-aligned box sampled onto a 256^3 FP32 SDF grid.
 
+sphere_block: one 128x128x64 FP32 lattice block (1,048,576 particles) at
+s = 0.01 m, seeded +-0.01 s jitter, in the origin corner of a 4 x 2 x 2 m box,
+with a sphere of radius 0.3 m at the box centre sampled onto a 128^3 FP32 SDF
+grid over the sphere's bounding box grown by 0.1 m.
+
 batched_scenes: independent scenes, each with a particle count log-uniform in
 [n_min, n_max], box aspect ratios x:z and y:z uniform in [1, 4], a fluid block
 filling a fraction uniform in [0.1, 0.5] of the box (a dam break in the
@@ -32,6 +37,20 @@ def double_dam_break(scale=1.0):
               sa.box_shape((3.0, 1.45, 0.25), (0.25, 0.2, 0.25))]
     sdf = sa.sdf_sample_shapes(shapes, dims, wmin, wmax)
     return dict(s=s, box=(6.0, 2.0, 2.0), pos=pos, sdf=sdf, dims=dims, wmin=wmin, wmax=wmax,
+                dx=(wmax[0] - wmin[0]) / dims[0])
+
+
+def sphere_block(scale=1.0):
+    s = float(np.float32(0.01))
+    nx, ny, nz = (max(1, int(round(128 * scale))), max(1, int(round(128 * scale))),
+                  max(1, int(round(64 * scale))))
+    pos = sa.scene_lattice([sa.Block(nx, ny, nz, 0.0, 0.0, 0.0)], s, jitter=0.01, seed=2024)
+    dims = (128, 128, 128)
+    c, r = (2.0, 1.0, 1.0), 0.3
+    wmin = tuple(ci - r - 0.1 for ci in c)
+    wmax = tuple(ci + r + 0.1 for ci in c)
+    sdf = sa.sdf_sample_shapes([sa.sphere(c, r)], dims, wmin, wmax)
+    return dict(s=s, box=(4.0, 2.0, 2.0), pos=pos, sdf=sdf, dims=dims, wmin=wmin, wmax=wmax,
                 dx=(wmax[0] - wmin[0]) / dims[0])
 
 

@@ -33,6 +33,12 @@ CASES = {
         workload="double_dam_break", scale=0.25, iters=4, substeps=200, every=20,
         physics=dict(k=0.5, c=0.01, lambda_eps=600.0, vort_eps=0.01, vort_model=1,
                      surface=True, viscosity=True, vorticity=True)),
+    # one 128x128x64 block (1M particles) at s = 0.01 m in a 4 x 2 x 2 m box with a
+    # sphere of radius 0.3 m at the centre, all terms
+    "sphere_block_1m": dict(
+        workload="sphere_block", scale=1.0, iters=4, substeps=150, every=15,
+        physics=dict(k=0.5, c=0.01, lambda_eps=600.0, vort_eps=0.01, vort_model=1,
+                     surface=True, viscosity=True, vorticity=True)),
     # the full 8M double dam break: after about 80 substeps the 1 m water columns
     # compress past what 4 Jacobi iterations hold, particles pile past the
     # 60-neighbour cap and the maximum error grows to ~10 (the statistics of the
@@ -53,7 +59,7 @@ def scene(case):
                                 seed=lt["seed"])
         return lt["s"], pos, case["box"], None
     from sphpc_amd import workloads
-    sc = workloads.double_dam_break(case["scale"])
+    sc = getattr(workloads, case["workload"])(case["scale"])
     return sc["s"], sc["pos"], sc["box"], sc
 
 
