@@ -94,15 +94,33 @@ def cpu_baseline(sc, substeps, threads):
                 density=np.zeros(n), lambda_=np.zeros(n),
                 nbr=np.zeros(n * orc.MAXN, np.uint32), cnt=np.zeros(n, np.int32))
     st = orc.OrcState(**{k: v.ctypes.data for k, v in arrs.items()})
+    L.orc_set_threads(threads)
     L.orc_substep(n, C.byref(st), C.byref(p), C.byref(ob), 4, 0.0, None)
     t0 = time.perf_counter()
     for _ in range(substeps):
         L.orc_substep(n, C.byref(st), C.byref(p), C.byref(ob), 4, 0.0, None)
     dt = time.perf_counter() - t0
+    # one thread (the reference's own CPU path is sequential), one substep of
+    # the first 2^20 particle ids of the same initial state
+    n1 = min(n, 1 << 20)
+    a1 = {k: (np.ascontiguousarray(v[:n1 * orc.MAXN]) if k == "nbr" else
+              np.ascontiguousarray(v[:n1])) for k, v in arrs.items()}
+    for k, v in zip(("x", "y", "z"), cols):
+        a1[k] = np.ascontiguousarray(v[:n1])
+    for k in ("vx", "vy", "vz"):
+        a1[k] = np.zeros(n1)
+    st1 = orc.OrcState(**{k: v.ctypes.data for k, v in a1.items()})
+    L.orc_set_threads(1)
+    t1 = time.perf_counter()
+    L.orc_substep(n1, C.byref(st1), C.byref(p), C.byref(ob), 4, 0.0, None)
+    dt1 = time.perf_counter() - t1
+    L.orc_set_threads(threads)
     return dict(value=n * substeps / dt, unit="particle-updates/s", cores=threads, kind="port",
+                one_thread_value=n1 / dt1,
                 sample=f"FP64 C oracle (OpenMP) on the same {n}-particle initial state: "
-                       f"1 warm-up + {substeps} timed substeps ({dt:.1f} s); "
-                       "reported baseline only")
+                       f"1 warm-up + {substeps} timed substeps on {threads} threads ({dt:.1f} s); "
+                       f"one_thread_value: 1 substep of its first {n1} particles on 1 thread "
+                       f"({dt1:.1f} s); reported baseline only")
 
 
 def timed_region(ctx, steps):

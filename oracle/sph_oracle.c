@@ -9,6 +9,9 @@
  * oracle/_ref by tests/test_oracle_vs_ref.py).  OpenMP replaces each
  * `#pragma acc parallel loop`.
  */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include "sph_oracle.h"
 
 #include <math.h>
@@ -671,6 +674,14 @@ double orc_max_density_error(int n, const double *density, double rest_density) 
 /* ---------------------------------------------------------------------- */
 /* Driver order, simulation.c:59-108                                         */
 /* ---------------------------------------------------------------------- */
+void orc_set_threads(int threads) {
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#else
+  (void)threads;
+#endif
+}
+
 int orc_substep(int n, orc_state *s, const orc_params *p, const orc_obstacle *o,
                 int iters, double tol, double *max_err_out) {
   const size_t N = (size_t)(n > 0 ? n : 1);

@@ -157,6 +157,8 @@ typedef struct orc_state {
   double *density, *lambda;           /* last Jacobi values (out) */
   uint32_t *nbr; int *cnt;            /* neighbour lists (out) */
 } orc_state;
+/* OpenMP threads of the following calls (timing only; results do not depend on it). */
+void orc_set_threads(int threads);
 int orc_substep(int n, orc_state *s, const orc_params *p, const orc_obstacle *o,
                 int iters, double tol, double *max_err_out);
 

@@ -108,6 +108,8 @@ def oracle():
         L.orc_update_positions.argtypes = [C.c_int] + [_dp] * 6
         L.orc_max_density_error.restype = C.c_double
         L.orc_max_density_error.argtypes = [C.c_int, _dp, C.c_double]
+        L.orc_set_threads.restype = None
+        L.orc_set_threads.argtypes = [C.c_int]
         L.orc_substep.restype = C.c_int
         L.orc_substep.argtypes = [C.c_int, C.POINTER(OrcState), P, C.POINTER(OrcObstacle),
                                   C.c_int, C.c_double, C.POINTER(C.c_double)]
