@@ -247,7 +247,6 @@ extern "C" void sph_default_params(sph_params *p, double h) {
 // Device buffers, stream and kernel set-up of a context whose grid (cells,
 // key_bits) and capacity are known.  Frees the context on failure.
 static int ctx_allocate(sph_ctx *c, int max_particles, sph_ctx **out) {
-  int rc = SPH_OK;
   auto fail = [&](int code) {
     free_all(c);
     delete c;
@@ -255,8 +254,6 @@ static int ctx_allocate(sph_ctx *c, int max_particles, sph_ctx **out) {
   };
   hipError_t e = hipGetDevice(&c->device);
   if (e != hipSuccess) return fail(hip_fail(e, "hipGetDevice"));
-  rc = configure_kernels();
-  if (rc) return fail(rc);
   e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) return fail(hip_fail(e, "hipStreamCreate"));
 

@@ -273,6 +273,5 @@ int launch_warm_samples(sph_ctx *c, float *x, float *t, uint32_t count, int ever
 int launch_warm_fallback(sph_ctx *c, const float4 *pre_warm);
 int launch_scene_errors(sph_ctx *c, uint32_t *dst = nullptr);
 int mlp_pack(sph_mlp *m, MlpPacked *dst, hipStream_t s);  // sph_mlp.hip
-int configure_kernels();  // LDS limits of the tiled kernels, once per process
 
 }  // namespace sph

@@ -1675,8 +1675,8 @@ int launch_surface(sph_ctx *c) {
   with_depth(tuned_depth(c, K_SURFACE), [&](auto D_) {
     constexpr int NBv = decltype(D_)::value;
     LAUNCH_TIMED(c, K_SURFACE, 0,
-               k_surface<NBv><<<nb, bs, 0, c->stream>>>(n, nb, c->pos, c->aux, c->list,
-                                                       c->list_head, c->dev, c->vel));
+                 k_surface<NBv><<<nb, bs, 0, c->stream>>>(n, nb, c->pos, c->aux, c->list,
+                                                         c->list_head, c->dev, c->vel));
   });
   SPH_HIP(hipGetLastError());
   return SPH_OK;
@@ -1687,7 +1687,7 @@ int launch_xsph(sph_ctx *c) {
   float4 *out = (c->vel == c->vel_a) ? c->vel_b : c->vel_a;
   if (n) {
     const uint32_t bs = tuned_block(c, K_XSPH);
-  const uint32_t nb = blocks_for(n, bs);
+    const uint32_t nb = blocks_for(n, bs);
     with_depth(tuned_depth(c, K_XSPH), [&](auto D_) {
       constexpr int NBv = decltype(D_)::value;
       LAUNCH_TIMED(c, K_XSPH, 0,
@@ -1708,8 +1708,9 @@ int launch_vorticity(sph_ctx *c) {
   with_depth(tuned_depth(c, K_VORTICITY), [&](auto D_) {
     constexpr int NBv = decltype(D_)::value;
     LAUNCH_TIMED(c, K_VORTICITY, 0,
-               k_vorticity<NBv><<<nb, bs, 0, c->stream>>>(n, nb, c->pos, c->vel, c->list,
-                                                         c->list_head, c->dev, c->aux, c->pw));
+                 k_vorticity<NBv><<<nb, bs, 0, c->stream>>>(n, nb, c->pos, c->vel, c->list,
+                                                           c->list_head, c->dev, c->aux,
+                                                           c->pw));
   });
   SPH_HIP(hipGetLastError());
   return SPH_OK;
@@ -1720,7 +1721,7 @@ int launch_confine(sph_ctx *c, bool commit) {
   float4 *out = (c->vel == c->vel_a) ? c->vel_b : c->vel_a;
   if (n) {
     const uint32_t bs = tuned_block(c, K_CONFINE);
-  const uint32_t nb = blocks_for(n, bs);
+    const uint32_t nb = blocks_for(n, bs);
     if (commit)
       with_depth(tuned_depth(c, K_CONFINE), [&](auto D_) {
         constexpr int NBv = decltype(D_)::value;
@@ -1742,8 +1743,6 @@ int launch_confine(sph_ctx *c, bool commit) {
   if (!commit) c->vel = out;
   return SPH_OK;
 }
-
-int configure_kernels() { return SPH_OK; }
 
 int launch_commit(sph_ctx *c) {
   const uint32_t n = c->n;
