@@ -108,7 +108,10 @@ int sph_get_device_count(int *count);
 void sph_default_params(sph_params *p, double smoothing_radius);
 
 /* --- lifecycle: AllocInitParticles/AllocInitNeighbors/AllocInitObstacle
- *     (particles.h:104-108, neighbors.h:35-40, obstacle.h:32-33) --------- */
+ *     (particles.h:104-108, neighbors.h:35-40, obstacle.h:32-33) ---------
+ * A context holds at most SPH_MAX_PARTICLES particles (a batch: all scenes
+ * together); its particle arrays are addressed with 32-bit byte offsets. */
+#define SPH_MAX_PARTICLES ((1 << 28) - 64)
 int sph_create(const sph_params *params, const sph_aabb *boundary, int max_particles,
                sph_ctx **out);
 int sph_destroy(sph_ctx *ctx);  /* FinalizeParticles/FinalizeNeighbors */

@@ -312,6 +312,9 @@ extern "C" int sph_create(const sph_params *params, const sph_aabb *boundary, in
   if (!(boundary->max_x > 0.0 && boundary->max_y > 0.0 && boundary->max_z > 0.0))
     return arg_error("boundary extent must be positive");
   if (max_particles < 0) return arg_error("max_particles must be >= 0");
+  if ((uint32_t)max_particles > kMaxParticles)
+    return arg_error("max_particles exceeds 2^28 - 64 (particle arrays are addressed with "
+                     "32-bit byte offsets)");
 
   sph_ctx *c = new sph_ctx();
   c->P = *params;
@@ -392,7 +395,9 @@ extern "C" int sph_create_batch(const sph_params *params, int nscenes, const sph
     first[k + 1] = (uint32_t)std::min<uint64_t>(total, 0xffffffffu);
     if (cells >= 0xffffffffull) return arg_error("hash grids exceed 2^32 cells");
   }
-  if (total > 0x7fffffffull) return arg_error("too many particles");
+  if (total > kMaxParticles)
+    return arg_error("too many particles: the scenes' total exceeds 2^28 - 64 (particle "
+                     "arrays are addressed with 32-bit byte offsets)");
   sph_ctx *c = new sph_ctx();
   c->P = *params;
   c->box = sph_aabb{0.0, ext[0], 0.0, ext[1], 0.0, ext[2]};

@@ -15,6 +15,10 @@ constexpr int kMaxNb = SPH_MAX_NEIGHBORS;
 constexpr int kBlock = 256;          // threads per block for elementwise kernels
 constexpr int kWave = 64;            // CDNA wavefront
 constexpr int kListStride = kWave * kMaxNb;  // list words per 64-particle group
+// Largest context: float4 particle arrays (plus their padding) stay below
+// 4 GiB, so that gathers address them with 32-bit byte offsets.
+constexpr uint32_t kMaxParticles = SPH_MAX_PARTICLES;
+static_assert((kMaxParticles + 64ull) * 16ull <= (1ull << 32), "32-bit byte offsets");
 // Neighbour-list head per particle: the first candidate index of each of the
 // three dz layers and kHeadCount | layer-0 count << 8 | layers-0..1 count << 16
 // | kHeadWide.  A compact list holds 16-bit offsets from the layer's first
