@@ -337,12 +337,10 @@ __global__ __launch_bounds__(kBlock) void k_predict(uint32_t n, const float4 *__
   xs[i] = make_float4(px, py, pz, 0.0f);
 }
 
-// HashParticles (neighbors.c:101-125) plus the cell histogram used for the
-// cell tables.  count must be zero on entry (the scan re-zeroes it).
+// HashParticles (neighbors.c:101-125).
 template <bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_hash(uint32_t n, const float4 *__restrict__ xs,
-                                                 uint32_t *__restrict__ keys,
-                                                 uint32_t *__restrict__ count, DevParams P,
+                                                 uint32_t *__restrict__ keys, DevParams P,
                                                  uint32_t *__restrict__ stats, BatchDev B) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
@@ -350,7 +348,6 @@ __global__ __launch_bounds__(kBlock) void k_hash(uint32_t n, const float4 *__res
   uint32_t clamped = 0;
   const uint32_t key = hash_key(a.x, a.y, a.z, P, geo_of<BATCH>(i, P, B), &clamped);
   keys[i] = key;
-  atomicAdd(&count[key], 1u);
   add_counter64(stats, STAT_CLAMPS, clamped);
 }
 
@@ -363,7 +360,6 @@ __global__ __launch_bounds__(kBlock) void k_predict_hash(uint32_t n,
                                                          const float4 *__restrict__ v,
                                                          float4 *__restrict__ xs,
                                                          uint32_t *__restrict__ keys,
-                                                         uint32_t *__restrict__ count,
                                                          DevParams P,
                                                          uint32_t *__restrict__ stats,
                                                          BatchDev B) {
@@ -378,21 +374,31 @@ __global__ __launch_bounds__(kBlock) void k_predict_hash(uint32_t n,
   uint32_t clamped = 0;
   const uint32_t key = hash_key(px, py, pz, P, g, &clamped);
   keys[i] = key;
-  atomicAdd(&count[key], 1u);
   add_counter64(stats, STAT_CLAMPS, clamped);
 }
 
-// Gather x* and x into (cell key, id) order.
+// Gather x* and x into (cell key, id) order, and the cell histogram for the
+// cell tables from the sorted keys: the first particle of each run of equal
+// keys writes the run's length (no atomics; the counts of empty cells stay
+// zero, as the scan leaves them).
 __global__ __launch_bounds__(kBlock) void k_reorder(uint32_t n, const uint32_t *__restrict__ id_s,
+                                                    const uint32_t *__restrict__ key_s,
                                                     const float4 *__restrict__ xs_can,
                                                     const float4 *__restrict__ x_can,
                                                     float4 *__restrict__ pos,
-                                                    float4 *__restrict__ xold) {
+                                                    float4 *__restrict__ xold,
+                                                    uint32_t *__restrict__ count) {
   const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
   if (s >= n) return;
   const uint32_t id = id_s[s];
   pos[s] = xs_can[id];
   xold[s] = x_can[id];
+  const uint32_t k = key_s[s];
+  if (s == 0 || key_s[s - 1] != k) {
+    uint32_t e = s + 1;
+    while (e < n && key_s[e] == k) ++e;
+    count[k] = e - s;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1381,21 +1387,21 @@ int launch_neighbor_search(sph_ctx *c, bool fused) {
     if (fused && c->nscenes) {
       LAUNCH_TIMED(c, K_PREDICT_HASH, 0,
                    k_predict_hash<true><<<blocks_for(n), kBlock, 0, s>>>(
-                       n, c->x_can, c->v_can, c->xs_can, c->keys, c->cell_count, c->dev,
+                       n, c->x_can, c->v_can, c->xs_can, c->keys, c->dev,
                        c->stats, B));
     } else if (fused) {
       LAUNCH_TIMED(c, K_PREDICT_HASH, 0,
                    k_predict_hash<false><<<blocks_for(n), kBlock, 0, s>>>(
-                       n, c->x_can, c->v_can, c->xs_can, c->keys, c->cell_count, c->dev,
+                       n, c->x_can, c->v_can, c->xs_can, c->keys, c->dev,
                        c->stats, B));
     } else if (c->nscenes) {
       LAUNCH_TIMED(c, K_HASH, 0,
                    k_hash<true><<<blocks_for(n), kBlock, 0, s>>>(
-                       n, c->xs_can, c->keys, c->cell_count, c->dev, c->stats, B));
+                       n, c->xs_can, c->keys, c->dev, c->stats, B));
     } else {
       LAUNCH_TIMED(c, K_HASH, 0,
                    k_hash<false><<<blocks_for(n), kBlock, 0, s>>>(
-                       n, c->xs_can, c->keys, c->cell_count, c->dev, c->stats, B));
+                       n, c->xs_can, c->keys, c->dev, c->stats, B));
     }
     SPH_HIP(hipGetLastError());
   }
@@ -1410,15 +1416,18 @@ int launch_neighbor_search(sph_ctx *c, bool fused) {
   }
   c->key_s = ko;
   c->id_s = vo;
+  if (n) {
+    c->pos = c->pos_a;
+    LAUNCH_TIMED(c, K_REORDER, 0,
+                 k_reorder<<<blocks_for(n), kBlock, 0, s>>>(n, c->id_s, c->key_s, c->xs_can,
+                                                            c->x_can, c->pos, c->xold,
+                                                            c->cell_count));
+    SPH_HIP(hipGetLastError());
+  }
   // cell_start[c] = #keys < c (lower bound), cell_start[cells] = n; re-zeroes counts
   rc = exclusive_scan(c->cell_count, c->cell_start, c->cells, true, true, &c->scan, s, c);
   if (rc) return rc;
   if (n) {
-    c->pos = c->pos_a;
-    LAUNCH_TIMED(c, K_REORDER, 0,
-                 k_reorder<<<blocks_for(n), kBlock, 0, s>>>(n, c->id_s, c->xs_can, c->x_can,
-                                                            c->pos, c->xold));
-    SPH_HIP(hipGetLastError());
     const uint32_t bs = tuned_block(c, K_BUILD_LISTS);
     const uint32_t nb = blocks_for(n, bs);
     if (c->nscenes)
