@@ -52,3 +52,28 @@ def test_defaults_and_argument_errors_without_gpu():
     box.min_x = 0.5
     assert lib.sph_create(C.byref(p), C.byref(box), 10, C.byref(h)) == -1
     assert b"origin" in lib.sph_last_error()
+
+
+def test_capacity_limit_without_gpu():
+    """Contexts are capped at SPH_MAX_PARTICLES (float4 particle arrays below
+    4 GiB, addressed with 32-bit byte offsets); the check precedes any device
+    call."""
+    import sphpc_amd as sa
+    text = open(HEADER).read()
+    m = re.search(r"#define SPH_MAX_PARTICLES \(\(1 << 28\) - 64\)", text)
+    assert m, "SPH_MAX_PARTICLES definition"
+    cap = (1 << 28) - 64
+    assert (cap + 64) * 16 <= 1 << 32
+    lib = sa.lib()
+    p = sa.default_params(0.03999999910593033)
+    p.rest_mass = 0.008
+    box = sa.AABB(0.0, 1.0, 0.0, 1.0, 0.0, 1.0)
+    h = C.c_void_p()
+    assert lib.sph_create(C.byref(p), C.byref(box), cap + 1, C.byref(h)) == -1
+    assert b"2^28" in lib.sph_last_error()
+    scenes = (sa.Scene * 2)()
+    for s in scenes:
+        s.particles = cap // 2 + 1
+        s.size_x = s.size_y = s.size_z = 1.0
+    assert lib.sph_create_batch(C.byref(p), 2, scenes, C.byref(h)) == -1
+    assert b"2^28" in lib.sph_last_error()
