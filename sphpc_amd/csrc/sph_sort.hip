@@ -166,16 +166,22 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_upsweep(const uint32_t *__
   __syncthreads();
   const int w = threadIdx.x >> 6;
   const uint64_t lt = lanemask_lt();
-  volatile uint32_t *hw = &h[w][0];
   const uint32_t base = blockIdx.x * RS_TILE;
-#pragma unroll 4
+  uint32_t kk[RS_ITEMS];
+#pragma unroll
+  for (int r = 0; r < RS_ITEMS; ++r) {
+    const uint32_t i = base + r * RS_THREADS + threadIdx.x;
+    kk[r] = i < n ? keys[i] : 0u;
+  }
+#pragma unroll
   for (int r = 0; r < RS_ITEMS; ++r) {
     const uint32_t i = base + r * RS_THREADS + threadIdx.x;
     const bool valid = i < n;
-    const uint32_t d = valid ? ((keys[i] >> shift) & 255u) : 0u;
+    const uint32_t d = (kk[r] >> shift) & 255u;
     const uint64_t peers = match_digit(d, valid);
-    // one leader per digit per wave: leaders of one wave touch distinct slots
-    if (valid && (peers & lt) == 0) hw[d] = hw[d] + (uint32_t)__popcll(peers);
+    // one leader per digit per wave adds the digit's count (a non-returning
+    // LDS add: no wait between rounds)
+    if (valid && (peers & lt) == 0) atomicAdd(&h[w][d], (uint32_t)__popcll(peers));
   }
   __syncthreads();
   uint32_t s = 0;
@@ -201,19 +207,27 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_downsweep(
   const uint32_t tile0 = blockIdx.x * RS_TILE;
   const uint32_t wbase = tile0 + (uint32_t)w * 64u * RS_ITEMS;
   uint32_t k[RS_ITEMS], v[RS_ITEMS], rk[RS_ITEMS];
-  volatile uint32_t *wc = &wcnt[w][0];
 #pragma unroll
   for (int r = 0; r < RS_ITEMS; ++r) {
     const uint32_t i = wbase + (uint32_t)r * 64u + (uint32_t)lane;
     const bool valid = i < n;
     k[r] = valid ? kin[i] : 0u;
     v[r] = valid ? (vin ? vin[i] : i) : 0u;
+  }
+#pragma unroll
+  for (int r = 0; r < RS_ITEMS; ++r) {
+    const uint32_t i = wbase + (uint32_t)r * 64u + (uint32_t)lane;
+    const bool valid = i < n;
     const uint32_t d = (k[r] >> shift) & 255u;
     const uint64_t peers = match_digit(d, valid);
     const uint32_t below = (uint32_t)__popcll(peers & lt);
-    const uint32_t old = valid ? wc[d] : 0u;
-    rk[r] = old + below;
-    if (valid && below == 0) wc[d] = old + (uint32_t)__popcll(peers);
+    // the digit's leader (lowest peer) takes the wave's running count and adds
+    // the peers; LDS operations of a wave complete in order, so later rounds
+    // see this one's add without waiting for its result
+    uint32_t old = 0u;
+    if (valid && below == 0) old = atomicAdd(&wcnt[w][d], (uint32_t)__popcll(peers));
+    const int leader = peers ? (int)__builtin_ctzll(peers) : lane;
+    rk[r] = __shfl(old, leader, 64) + below;
   }
   __syncthreads();
   // Digit `tid`: prefix over waves, then tile-level digit starts.
