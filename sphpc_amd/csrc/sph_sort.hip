@@ -71,17 +71,35 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
 // --------------------------------------------------------------------------
 // Scan
 // --------------------------------------------------------------------------
+// Thread t of a tile owns entries [t*SC_ITEMS, (t+1)*SC_ITEMS): four 16-byte
+// loads and stores per thread, one block scan per tile.
+__device__ __forceinline__ void load_items(const uint32_t *in, uint32_t i0, uint32_t len,
+                                           uint32_t (&x)[SC_ITEMS]) {
+  if (i0 + SC_ITEMS <= len) {
+    const uint4 *v = reinterpret_cast<const uint4 *>(in + i0);
+#pragma unroll
+    for (int q = 0; q < SC_ITEMS / 4; ++q) {
+      const uint4 a = v[q];
+      x[4 * q] = a.x;
+      x[4 * q + 1] = a.y;
+      x[4 * q + 2] = a.z;
+      x[4 * q + 3] = a.w;
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < SC_ITEMS; ++r) x[r] = i0 + r < len ? in[i0 + r] : 0u;
+  }
+}
+
 __global__ __launch_bounds__(SC_THREADS) void k_scan_tile_sums(const uint32_t *__restrict__ in,
                                                                uint32_t len,
                                                                uint32_t *__restrict__ sums) {
   __shared__ uint32_t wtot[4];
-  const uint32_t base = blockIdx.x * SC_TILE;
+  uint32_t x[SC_ITEMS];
+  load_items(in, blockIdx.x * SC_TILE + threadIdx.x * SC_ITEMS, len, x);
   uint32_t s = 0;
 #pragma unroll
-  for (int r = 0; r < SC_ITEMS; ++r) {
-    const uint32_t i = base + r * SC_THREADS + threadIdx.x;
-    if (i < len) s += in[i];
-  }
+  for (int r = 0; r < SC_ITEMS; ++r) s += x[r];
   uint32_t tot;
   block_excl_scan_256(s, wtot, &tot);
   if (threadIdx.x == 0) sums[blockIdx.x] = tot;
@@ -106,20 +124,40 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_apply(uint32_t *in, uint32_
                                                            const uint32_t *__restrict__ sums,
                                                            int zero_in, int write_total) {
   __shared__ uint32_t wtot[4];
-  const uint32_t base = blockIdx.x * SC_TILE;
-  uint32_t carry = sums[blockIdx.x];
+  const uint32_t i0 = blockIdx.x * SC_TILE + threadIdx.x * SC_ITEMS;
+  uint32_t x[SC_ITEMS];
+  load_items(in, i0, len, x);
+  uint32_t own = 0;
+#pragma unroll
+  for (int r = 0; r < SC_ITEMS; ++r) own += x[r];
+  uint32_t tot;
+  uint32_t run = sums[blockIdx.x] + block_excl_scan_256(own, wtot, &tot);
+#pragma unroll
   for (int r = 0; r < SC_ITEMS; ++r) {
-    const uint32_t i = base + r * SC_THREADS + threadIdx.x;
-    const uint32_t x = i < len ? in[i] : 0u;
-    uint32_t tot;
-    const uint32_t e = block_excl_scan_256(x, wtot, &tot);
-    if (i < len) {
-      out[i] = carry + e;
-      if (zero_in) in[i] = 0u;
-    }
-    carry += tot;
+    const uint32_t e = run;
+    run += x[r];
+    x[r] = e;
   }
-  if (write_total && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[len] = carry;
+  if (i0 + SC_ITEMS <= len) {
+    uint4 *o = reinterpret_cast<uint4 *>(out + i0);
+#pragma unroll
+    for (int q = 0; q < SC_ITEMS / 4; ++q)
+      o[q] = make_uint4(x[4 * q], x[4 * q + 1], x[4 * q + 2], x[4 * q + 3]);
+    if (zero_in) {
+      uint4 *z = reinterpret_cast<uint4 *>(in + i0);
+#pragma unroll
+      for (int q = 0; q < SC_ITEMS / 4; ++q) z[q] = make_uint4(0u, 0u, 0u, 0u);
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < SC_ITEMS; ++r)
+      if (i0 + r < len) {
+        out[i0 + r] = x[r];
+        if (zero_in) in[i0 + r] = 0u;
+      }
+  }
+  // the thread holding entry len-1 writes the total after it
+  if (write_total && i0 < len && len <= i0 + SC_ITEMS) out[len] = run;
 }
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, uint32_t len) {
@@ -135,6 +173,10 @@ int exclusive_scan(uint32_t *in, uint32_t *out, uint32_t len, bool zero_in, bool
       SPH_HIP(hipGetLastError());
     }
     return SPH_OK;
+  }
+  if (((uintptr_t)in | (uintptr_t)out) & 15u) {
+    set_error("exclusive_scan: buffers must be 16-byte aligned");
+    return SPH_ERR_ARG;
   }
   const uint32_t tiles = (len + SC_TILE - 1) / SC_TILE;
   if (sc->tile_cap < tiles) {
