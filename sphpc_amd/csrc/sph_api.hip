@@ -171,7 +171,7 @@ static void graph_reset(sph_ctx *c) {
 static void free_all(sph_ctx *c) {
   graph_reset(c);
   void *bufs[] = {c->x_can, c->v_can, c->xs_can, c->keys, c->keys_alt, c->ids, c->ids_alt,
-                  c->hist, c->cell_count, c->cell_start, c->pos_a, c->pos_b, c->xold,
+                  c->hist, c->cell_end, c->cell_start, c->pos_a, c->pos_b, c->xold,
                   c->vel_a, c->vel_b, c->aux, c->dpbuf, c->rho, c->list, c->list_head,
                   c->stats, c->pair_partial, c->red, c->sdf_buf, c->scan.tile_sums,
                   c->pw, c->lam_acc, c->lam_prev, c->feat, c->scenes_dev, c->scene_of,
@@ -268,7 +268,7 @@ static int ctx_allocate(sph_ctx *c, int max_particles, sph_ctx **out) {
       {(void **)&c->x_can, f4},       {(void **)&c->v_can, f4},
       {(void **)&c->xs_can, f4},      {(void **)&c->keys, u4},
       {(void **)&c->keys_alt, u4},    {(void **)&c->ids, u4},
-      {(void **)&c->ids_alt, u4},     {(void **)&c->cell_count, sizeof(uint32_t) * (c->cells + 1ull)},
+      {(void **)&c->ids_alt, u4},     {(void **)&c->cell_end, sizeof(uint32_t) * (c->cells + 1ull)},
       {(void **)&c->cell_start, sizeof(uint32_t) * (c->cells + 1ull)},
       {(void **)&c->pos_a, f4 + sizeof(float4) * kPosPad},
       {(void **)&c->pos_b, f4 + sizeof(float4) * kPosPad},
@@ -286,7 +286,7 @@ static int ctx_allocate(sph_ctx *c, int max_particles, sph_ctx **out) {
     e = hipMalloc(a.p, a.bytes);
     if (e != hipSuccess) return fail(hip_fail(e, "hipMalloc (context buffers)"));
   }
-  e = hipMemsetAsync(c->cell_count, 0, sizeof(uint32_t) * (c->cells + 1ull), c->stream);
+  e = hipMemsetAsync(c->cell_end, 0, sizeof(uint32_t) * (c->cells + 1ull), c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(c->stats, 0, sizeof(uint32_t) * STAT_WORDS, c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(c->list_head, 0, sizeof(uint4) * cap, c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(c->pos_a, 0, f4 + sizeof(float4) * kPosPad, c->stream);

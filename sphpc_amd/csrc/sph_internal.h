@@ -164,7 +164,9 @@ struct sph_ctx {
   uint32_t *hist = nullptr;
   size_t hist_cap = 0;
   const uint32_t *key_s = nullptr, *id_s = nullptr;  // final sorted buffers
-  uint32_t *cell_count = nullptr, *cell_start = nullptr;  // cells + 1 entries
+  // cells + 1 entries; cell_end holds the run ends of the sorted keys, zero
+  // between searches
+  uint32_t *cell_end = nullptr, *cell_start = nullptr;
   sph::ScanScratch scan;
   // sorted working set
   float4 *pos_a = nullptr, *pos_b = nullptr, *pos = nullptr;  // pos: current x*
@@ -242,9 +244,10 @@ inline uint32_t blocks_for(uint32_t n, uint32_t per = kBlock) { return (n + per 
 void prof_begin(sph_ctx *c, int kernel, hipEvent_t *a);
 void prof_end(sph_ctx *c, int kernel, hipEvent_t a);
 
-// Sort / scan (sph_sort.hip).
+// Sort / scan (sph_sort.hip).  out[i] = in[0] + ... + in[i-1] (max_op: the
+// maximum of those, 0 for none); write_total also sets out[len].
 int exclusive_scan(uint32_t *in, uint32_t *out, uint32_t len, bool zero_in, bool write_total,
-                   ScanScratch *scratch, hipStream_t s, sph_ctx *prof);
+                   ScanScratch *scratch, hipStream_t s, sph_ctx *prof, bool max_op = false);
 int radix_sort_pairs(const uint32_t *kin, const uint32_t *vin, uint32_t *k0, uint32_t *v0,
                      uint32_t *k1, uint32_t *v1, uint32_t n, int key_bits, uint32_t **hist,
                      size_t *hist_cap, ScanScratch *scratch, hipStream_t s, sph_ctx *prof,

@@ -377,28 +377,25 @@ __global__ __launch_bounds__(kBlock) void k_predict_hash(uint32_t n,
   add_counter64(stats, STAT_CLAMPS, clamped);
 }
 
-// Gather x* and x into (cell key, id) order, and the cell histogram for the
-// cell tables from the sorted keys: the first particle of each run of equal
-// keys writes the run's length (no atomics; the counts of empty cells stay
-// zero, as the scan leaves them).
+// Gather x* and x into (cell key, id) order, and the cell ends for the cell
+// tables from the sorted keys: the last particle of each run of equal keys
+// writes end[key] = its index + 1 (no atomics; the entries of empty cells stay
+// zero, as the scan leaves them).  The lower bound of cell c is then the
+// largest end of the cells before it (an exclusive max-scan).
 __global__ __launch_bounds__(kBlock) void k_reorder(uint32_t n, const uint32_t *__restrict__ id_s,
                                                     const uint32_t *__restrict__ key_s,
                                                     const float4 *__restrict__ xs_can,
                                                     const float4 *__restrict__ x_can,
                                                     float4 *__restrict__ pos,
                                                     float4 *__restrict__ xold,
-                                                    uint32_t *__restrict__ count) {
+                                                    uint32_t *__restrict__ cell_end) {
   const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
   if (s >= n) return;
   const uint32_t id = id_s[s];
   pos[s] = xs_can[id];
   xold[s] = x_can[id];
   const uint32_t k = key_s[s];
-  if (s == 0 || key_s[s - 1] != k) {
-    uint32_t e = s + 1;
-    while (e < n && key_s[e] == k) ++e;
-    count[k] = e - s;
-  }
+  if (s + 1 == n || key_s[s + 1] != k) cell_end[k] = s + 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -1421,11 +1418,12 @@ int launch_neighbor_search(sph_ctx *c, bool fused) {
     LAUNCH_TIMED(c, K_REORDER, 0,
                  k_reorder<<<blocks_for(n), kBlock, 0, s>>>(n, c->id_s, c->key_s, c->xs_can,
                                                             c->x_can, c->pos, c->xold,
-                                                            c->cell_count));
+                                                            c->cell_end));
     SPH_HIP(hipGetLastError());
   }
-  // cell_start[c] = #keys < c (lower bound), cell_start[cells] = n; re-zeroes counts
-  rc = exclusive_scan(c->cell_count, c->cell_start, c->cells, true, true, &c->scan, s, c);
+  // cell_start[c] = #keys < c (lower bound) = max end of the cells before c,
+  // cell_start[cells] = n; re-zeroes the ends
+  rc = exclusive_scan(c->cell_end, c->cell_start, c->cells, true, true, &c->scan, s, c, true);
   if (rc) return rc;
   if (n) {
     const uint32_t bs = tuned_block(c, K_BUILD_LISTS);

@@ -22,33 +22,44 @@ constexpr int SC_THREADS = 256;
 constexpr int SC_ITEMS = 16;
 constexpr int SC_TILE = SC_THREADS * SC_ITEMS;
 
+// Scan operators (identity 0 for both: the values are unsigned).
+struct OpAdd {
+  static __device__ __forceinline__ uint32_t f(uint32_t a, uint32_t b) { return a + b; }
+};
+struct OpMax {
+  static __device__ __forceinline__ uint32_t f(uint32_t a, uint32_t b) { return max(a, b); }
+};
+
+template <class Op = OpAdd>
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
   const int lane = threadIdx.x & 63;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
     const uint32_t y = __shfl_up(x, o, 64);
-    if (lane >= o) x += y;
+    if (lane >= o) x = Op::f(x, y);
   }
   return x;
 }
 
 // Exclusive scan over a 256-thread block; *tot receives the block total.
+template <class Op = OpAdd>
 __device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t x, uint32_t *wtot,
                                                         uint32_t *tot) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const uint32_t inc = wave_incl_scan(x);
+  const uint32_t inc = wave_incl_scan<Op>(x);
   if (lane == 63) wtot[w] = inc;
   __syncthreads();
   uint32_t pre = 0, all = 0;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const uint32_t t = wtot[i];
-    pre += (i < w) ? t : 0u;
-    all += t;
+    if (i < w) pre = Op::f(pre, t);
+    all = Op::f(all, t);
   }
   __syncthreads();
   *tot = all;
-  return pre + inc - x;
+  const uint32_t before = __shfl_up(inc, 1, 64);
+  return Op::f(pre, lane ? before : 0u);
 }
 
 // Lanes of the wave holding the same 8-bit digit (valid lanes only).
@@ -91,6 +102,7 @@ __device__ __forceinline__ void load_items(const uint32_t *in, uint32_t i0, uint
   }
 }
 
+template <class Op>
 __global__ __launch_bounds__(SC_THREADS) void k_scan_tile_sums(const uint32_t *__restrict__ in,
                                                                uint32_t len,
                                                                uint32_t *__restrict__ sums) {
@@ -99,12 +111,13 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_tile_sums(const uint32_t *_
   load_items(in, blockIdx.x * SC_TILE + threadIdx.x * SC_ITEMS, len, x);
   uint32_t s = 0;
 #pragma unroll
-  for (int r = 0; r < SC_ITEMS; ++r) s += x[r];
+  for (int r = 0; r < SC_ITEMS; ++r) s = Op::f(s, x[r]);
   uint32_t tot;
-  block_excl_scan_256(s, wtot, &tot);
+  block_excl_scan_256<Op>(s, wtot, &tot);
   if (threadIdx.x == 0) sums[blockIdx.x] = tot;
 }
 
+template <class Op>
 __global__ __launch_bounds__(SC_THREADS) void k_scan_spine(uint32_t *sums, uint32_t tiles) {
   __shared__ uint32_t wtot[4];
   uint32_t carry = 0;
@@ -112,13 +125,14 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_spine(uint32_t *sums, uint3
     const uint32_t i = base + threadIdx.x;
     const uint32_t x = i < tiles ? sums[i] : 0u;
     uint32_t tot;
-    const uint32_t e = block_excl_scan_256(x, wtot, &tot);
-    if (i < tiles) sums[i] = carry + e;
-    carry += tot;
+    const uint32_t e = block_excl_scan_256<Op>(x, wtot, &tot);
+    if (i < tiles) sums[i] = Op::f(carry, e);
+    carry = Op::f(carry, tot);
   }
 }
 
 // in and out may alias (in-place scan).
+template <class Op>
 __global__ __launch_bounds__(SC_THREADS) void k_scan_apply(uint32_t *in, uint32_t *out,
                                                            uint32_t len,
                                                            const uint32_t *__restrict__ sums,
@@ -129,13 +143,13 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_apply(uint32_t *in, uint32_
   load_items(in, i0, len, x);
   uint32_t own = 0;
 #pragma unroll
-  for (int r = 0; r < SC_ITEMS; ++r) own += x[r];
+  for (int r = 0; r < SC_ITEMS; ++r) own = Op::f(own, x[r]);
   uint32_t tot;
-  uint32_t run = sums[blockIdx.x] + block_excl_scan_256(own, wtot, &tot);
+  uint32_t run = Op::f(sums[blockIdx.x], block_excl_scan_256<Op>(own, wtot, &tot));
 #pragma unroll
   for (int r = 0; r < SC_ITEMS; ++r) {
     const uint32_t e = run;
-    run += x[r];
+    run = Op::f(run, x[r]);
     x[r] = e;
   }
   if (i0 + SC_ITEMS <= len) {
@@ -166,7 +180,7 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, uint32_t len) {
 }
 
 int exclusive_scan(uint32_t *in, uint32_t *out, uint32_t len, bool zero_in, bool write_total,
-                   ScanScratch *sc, hipStream_t s, sph_ctx *prof) {
+                   ScanScratch *sc, hipStream_t s, sph_ctx *prof, bool max_op) {
   if (len == 0) {
     if (write_total) {
       k_fill_u32<<<1, 64, 0, s>>>(out, 0u, 1u);
@@ -187,10 +201,17 @@ int exclusive_scan(uint32_t *in, uint32_t *out, uint32_t len, bool zero_in, bool
   }
   hipEvent_t ev = nullptr;
   if (prof) prof_begin(prof, K_SCAN, &ev);
-  k_scan_tile_sums<<<tiles, SC_THREADS, 0, s>>>(in, len, sc->tile_sums);
-  k_scan_spine<<<1, SC_THREADS, 0, s>>>(sc->tile_sums, tiles);
-  k_scan_apply<<<tiles, SC_THREADS, 0, s>>>(in, out, len, sc->tile_sums, zero_in ? 1 : 0,
-                                            write_total ? 1 : 0);
+  if (max_op) {
+    k_scan_tile_sums<OpMax><<<tiles, SC_THREADS, 0, s>>>(in, len, sc->tile_sums);
+    k_scan_spine<OpMax><<<1, SC_THREADS, 0, s>>>(sc->tile_sums, tiles);
+    k_scan_apply<OpMax><<<tiles, SC_THREADS, 0, s>>>(in, out, len, sc->tile_sums,
+                                                     zero_in ? 1 : 0, write_total ? 1 : 0);
+  } else {
+    k_scan_tile_sums<OpAdd><<<tiles, SC_THREADS, 0, s>>>(in, len, sc->tile_sums);
+    k_scan_spine<OpAdd><<<1, SC_THREADS, 0, s>>>(sc->tile_sums, tiles);
+    k_scan_apply<OpAdd><<<tiles, SC_THREADS, 0, s>>>(in, out, len, sc->tile_sums,
+                                                     zero_in ? 1 : 0, write_total ? 1 : 0);
+  }
   if (prof) prof_end(prof, K_SCAN, ev);
   SPH_HIP(hipGetLastError());
   return SPH_OK;
