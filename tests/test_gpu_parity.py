@@ -124,6 +124,40 @@ def test_neighbor_search_bit_exact(spread, seed, wide):
         assert cnt.max() == orc.MAXN and ol["dropped"] > 0
 
 
+def test_cell_tables_at_the_grid_ends_and_a_crowded_cell():
+    """Cell tables from the sorted keys' run ends: runs in the first and the
+    last cell of the grid, particles clamped onto the box faces from outside,
+    one cell holding 300 particles (five times the list cap), and long stretches
+    of empty cells between them."""
+    s = 0.02
+    sc = Scene(s, (1.0, 1.0, 1.0))
+    rng = np.random.default_rng(11)
+    h = 2 * s
+    crowd = 13 * h + rng.uniform(0.1 * h, 0.9 * h, (300, 3))  # inside cell (13, 13, 13)
+    low = rng.uniform(0.0, 0.5 * h, (40, 3))
+    high = 1.0 - rng.uniform(0.2 * h, 0.5 * h, (40, 3))
+    outside = np.concatenate([rng.uniform(-0.5, -0.1, (20, 3)), rng.uniform(1.1, 1.5, (20, 3))])
+    pos = np.concatenate([crowd, low, high, outside]).astype(np.float32)
+    pos = pos[rng.permutation(len(pos))]
+    vel = np.zeros_like(pos)
+    ctx = _staged_search(sc, pos, vel)
+    xs_gpu = ctx.field(sa.FIELD_POSITION_STAR)
+    xs, ys, zs = _soa(xs_gpu)
+    ol = oracle_lists(sc.o, xs, ys, zs)
+    k, i = ctx.sorted_pairs()
+    assert np.array_equal(k, ol["keys"]) and np.array_equal(i, ol["ids"])
+    nx, ny, nz = ctx.grid_dims()
+    # the origin cell, and the cell holding the clamped far corner of the box
+    c = int((1.0 - 1e-4 * h) // h)
+    assert k[0] == 0 and k[-1] == c + c * nx + c * nx * ny
+    st, en = ctx.cell_bounds()
+    assert np.array_equal(st, ol["start"]) and np.array_equal(en, ol["end"])
+    assert (en - st).max() == 300
+    ids, cnt = ctx.neighbors()
+    assert np.array_equal(cnt, ol["cnt"])
+    assert cnt.max() == orc.MAXN and ctx.stats().neighbor_overflow == ol["dropped"]
+
+
 def test_neighbor_search_long_layers_bit_exact():
     """A bar 8400 particles long and 3 cells thick: a dz layer of the middle
     cell row spans more than 2^16 sorted indices, so those wavefronts (and
