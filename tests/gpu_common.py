@@ -10,12 +10,16 @@ import numpy as np
 import orc
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ERR_LOG = os.path.join(ROOT, "gpurun_out", "parity_errors.jsonl")
+# measured GPU-vs-oracle errors are appended to the JSON-lines file this
+# variable names (no log when unset)
+ERR_LOG = os.environ.get("SPH_PARITY_LOG")
 
 
 def record(name, **vals):
     """Append measured GPU-vs-oracle errors (stated per quantity)."""
-    os.makedirs(os.path.dirname(ERR_LOG), exist_ok=True)
+    if not ERR_LOG:
+        return
+    os.makedirs(os.path.dirname(os.path.abspath(ERR_LOG)), exist_ok=True)
     with open(ERR_LOG, "a") as f:
         f.write(json.dumps({"test": name, **{k: float(v) for k, v in vals.items()}}) + "\n")
 
