@@ -21,7 +21,7 @@ typedef struct {
   char value[256];
 } entry;
 
-static char g_msg[512];
+static __thread char g_msg[512];  /* per thread, like sph_last_error */
 
 const char *sph_config_error(void) { return g_msg; }
 
@@ -63,7 +63,7 @@ static int get_int(const entry *e, int n, const char *key, int *out, int require
   double d = *out;
   int rc = get_double(e, n, key, &d, required);
   if (rc) return rc;
-  if (d != floor(d)) {
+  if (d != floor(d) || !(d >= -2147483648.0 && d <= 2147483647.0)) {
     snprintf(g_msg, sizeof(g_msg), "Aborting: conversion of data to type \"i\" failed (%s)",
              key);
     return SPH_ERR_ARG;
@@ -71,6 +71,8 @@ static int get_int(const entry *e, int n, const char *key, int *out, int require
   *out = (int)d;
   return SPH_OK;
 }
+
+static int parse_entries(const entry *E, int n, sph_run_config *cfg);
 
 int sph_config_load(const char *path, sph_run_config *cfg) {
   g_msg[0] = 0;
@@ -80,7 +82,12 @@ int sph_config_load(const char *path, sph_run_config *cfg) {
     snprintf(g_msg, sizeof(g_msg), "Aborting: %s: cannot open file", path);
     return SPH_ERR_ARG;
   }
-  static entry E[MAX_ENTRIES];
+  entry *E = (entry *)calloc(MAX_ENTRIES, sizeof(entry));
+  if (!E) {
+    fclose(f);
+    snprintf(g_msg, sizeof(g_msg), "Aborting: out of memory reading %s", path);
+    return SPH_ERR_NOMEM;
+  }
   int n = 0;
   char section[64] = "";
   char line[512];
@@ -110,7 +117,12 @@ int sph_config_load(const char *path, sph_run_config *cfg) {
     ++n;
   }
   fclose(f);
+  const int rc_all = parse_entries(E, n, cfg);
+  free(E);
+  return rc_all;
+}
 
+static int parse_entries(const entry *E, int n, sph_run_config *cfg) {
   memset(cfg, 0, sizeof(*cfg));
   int rc = 0;
   rc |= get_int(E, n, "SimParameters.number_steps", &cfg->number_steps, 1);
