@@ -13,7 +13,9 @@ One step = one substep of the solver over all particles with the state
 resident in HBM, replayed from a captured HIP graph (--no-graph: eager
 launches).  Prints one JSON line: the throughput, the roofline of the dominant
 kernel (the same substeps launched eagerly, every kernel event-timed on the
-library's stream), Jacobi iterations and error to 0.1% density error in
+library's stream), the VALU floor of each gather kernel (VALU instructions per
+launch from the committed rocprofv3 SQ_INSTS_VALU summary), Jacobi iterations
+and error to 0.1% density error in
 tolerance mode, plain and warm-started (trained network file), and the FP64
 CPU oracle timed on the host cores on a bounded sample of the same state
 (reported baseline only).  --workload batch times the batched-scene workload.
@@ -35,6 +37,8 @@ HBM_PEAK_GBS = 8000.0
 
 # Algorithmic bytes per particle of each kernel (FP32, each stream once per
 # pass, 3-vectors 12 B, scalars 4 B, a neighbour list 4(nbar+1) B).
+VALU_SIMDS = 256 * 4        # MI355X: 256 CUs x 4 SIMDs
+VALU_CLOCK_HZ = 2.4e9       # peak engine clock
 KERNEL_BYTES = {
     "density_lambda": lambda L: 20 + L,   # x* 12, list -> rho 4, lambda 4
     "dp_update": lambda L: 28 + L,        # x* 12, lambda 4, list -> x*' 12
@@ -73,6 +77,42 @@ def read_traffic(kernel, workload_tag):
                 if tok.startswith("bytes_per_launch="):
                     return float(tok.split("=", 1)[1])
     return None
+
+
+def read_valu(kernel, workload_tag):
+    """VALU instructions per launch from the committed rocprofv3 --pmc summary, or None."""
+    path = os.path.join(ROOT, "profiles", f"pmc_valu_{workload_tag}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if parts and parts[0] == kernel:
+            for tok in parts[1:]:
+                if tok.startswith("valu_insts_per_launch="):
+                    return float(tok.split("=", 1)[1])
+    return None
+
+
+def valu_ceiling(times, n, nbar, tag):
+    """Per gather kernel: the time its VALU instructions take at full issue rate
+    (wave64 instructions x 4 cycles over 1024 SIMDs at 2.4 GHz; a lower bound,
+    transcendentals issue at quarter rate), against its measured launch time, and
+    the HBM roofline fraction the kernel would reach at that VALU floor.  Where
+    that fraction is below 0.6 the 60% target is out of reach for the kernel."""
+    L = 4.0 * (nbar + 1.0)
+    out = {}
+    for k, (ms, cnt) in times.items():
+        if k not in KERNEL_BYTES:
+            continue
+        insts = read_valu(k, tag)
+        if insts is None:
+            continue
+        floor_s = insts * 4.0 / (VALU_SIMDS * VALU_CLOCK_HZ)
+        frac = KERNEL_BYTES[k](L) * n / floor_s / (HBM_PEAK_GBS * 1e9)
+        out[k] = {"valu_insts_per_launch": insts, "valu_floor_ms": floor_s * 1e3,
+                  "avg_launch_ms": ms / cnt, "valu_busy": floor_s * 1e3 / (ms / cnt),
+                  "hbm_frac_at_valu_floor": frac, "target_0p6_out_of_reach": frac < 0.6}
+    return out
 
 
 def cpu_baseline(sc, substeps, threads):
@@ -317,6 +357,8 @@ def main():
     }
     out["roofline"], out["kernels_ms_per_substep"] = roofline(
         times, n, nbar, prof_steps, "pbf_substep_8M" if args.scale == 1.0 else f"pbf_substep_scale{args.scale}")
+    out["valu_ceiling"] = valu_ceiling(
+        times, n, nbar, "pbf_substep_8M" if args.scale == 1.0 else f"pbf_substep_scale{args.scale}")
     out["config"]["graph"] = not args.no_graph
     out["config"]["autotune"] = compact_tuning(tuning)
     if not args.no_extras:
