@@ -179,6 +179,8 @@ static void free_all(sph_ctx *c) {
                   c->warm_net};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
+  if (c->io_dev) (void)hipFree(c->io_dev);
+  if (c->io_host) (void)hipHostFree(c->io_host);
   for (auto &p : c->prof.pending) {
     (void)hipEventDestroy(p.a);
     (void)hipEventDestroy(p.b);
@@ -549,6 +551,53 @@ extern "C" int sph_set_obstacle(sph_ctx *c, const sph_obstacle_desc *o) {
   return rc;
 }
 
+// SoA <-> the float4 canonical state, on the device (coalesced on both sides).
+__global__ void k_pack_state(uint32_t n, const float *__restrict__ soa,
+                             float4 *__restrict__ x, float4 *__restrict__ v) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  x[i] = make_float4(soa[i], soa[n + i], soa[2 * (size_t)n + i], 0.0f);
+  v[i] = make_float4(soa[3 * (size_t)n + i], soa[4 * (size_t)n + i], soa[5 * (size_t)n + i],
+                     0.0f);
+}
+
+__global__ void k_unpack_state(uint32_t n, const float4 *__restrict__ x,
+                               const float4 *__restrict__ v, float *__restrict__ soa,
+                               int with_velocity) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 a = x[i];
+  soa[i] = a.x;
+  soa[n + i] = a.y;
+  soa[2 * (size_t)n + i] = a.z;
+  if (with_velocity) {
+    const float4 b = v[i];
+    soa[3 * (size_t)n + i] = b.x;
+    soa[4 * (size_t)n + i] = b.y;
+    soa[5 * (size_t)n + i] = b.z;
+  }
+}
+
+// Device and pinned host staging for state transfers (6 floats per particle of
+// the capacity), allocated on first use.
+static int io_buffers(sph_ctx *c) {
+  if (c->io_dev && c->io_host) return SPH_OK;
+  const size_t bytes = sizeof(float) * 6 * (size_t)std::max(c->cap, 1);
+  if (!c->io_dev) {
+    hipError_t e = hipMalloc(&c->io_dev, bytes);
+    if (e != hipSuccess) {
+      c->io_dev = nullptr;
+      return hip_fail(e, "hipMalloc (state transfer staging)");
+    }
+  }
+  hipError_t e = hipHostMalloc(&c->io_host, bytes, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    c->io_host = nullptr;
+    return hip_fail(e, "hipHostMalloc (state transfer staging)");
+  }
+  return SPH_OK;
+}
+
 extern "C" int sph_upload_state(sph_ctx *c, int n, const float *x, const float *y,
                                 const float *z, const float *vx, const float *vy,
                                 const float *vz) {
@@ -562,17 +611,18 @@ extern "C" int sph_upload_state(sph_ctx *c, int n, const float *x, const float *
   if (c->nscenes && n != c->cap)
     return arg_error("a batched context takes all scenes' particles at once, in scene order");
   if (n != c->n) graph_reset(c);
-  std::vector<float4> hx((size_t)n), hv((size_t)n);
-  for (int i = 0; i < n; ++i) {
-    hx[i] = make_float4(x[i], y[i], z[i], 0.0f);
-    hv[i] = make_float4(vx[i], vy[i], vz[i], 0.0f);
-  }
   SPH_HIP(hipStreamSynchronize(c->stream));
   if (n) {
-    SPH_HIP(hipMemcpyAsync(c->x_can, hx.data(), sizeof(float4) * n, hipMemcpyHostToDevice,
-                           c->stream));
-    SPH_HIP(hipMemcpyAsync(c->v_can, hv.data(), sizeof(float4) * n, hipMemcpyHostToDevice,
-                           c->stream));
+    int rc = io_buffers(c);
+    if (rc) return rc;
+    const size_t un = (size_t)n;
+    const float *src[6] = {x, y, z, vx, vy, vz};
+    for (int k = 0; k < 6; ++k) memcpy(c->io_host + k * un, src[k], sizeof(float) * un);
+    SPH_HIP(hipMemcpyAsync(c->io_dev, c->io_host, sizeof(float) * 6 * un,
+                           hipMemcpyHostToDevice, c->stream));
+    k_pack_state<<<blocks_for((uint32_t)n), kBlock, 0, c->stream>>>((uint32_t)n, c->io_dev,
+                                                                    c->x_can, c->v_can);
+    SPH_HIP(hipGetLastError());
   }
   if (c->lam_prev) SPH_HIP(hipMemsetAsync(c->lam_prev, 0, sizeof(float) * c->cap, c->stream));
   SPH_HIP(hipStreamSynchronize(c->stream));
@@ -589,22 +639,22 @@ extern "C" int sph_download_state(sph_ctx *c, float *x, float *y, float *z, floa
   int rc = require_state(c);
   if (rc) return rc;
   const int n = c->n;
-  std::vector<float4> hx((size_t)n), hv((size_t)n);
-  if (n) {
-    SPH_HIP(hipMemcpyAsync(hx.data(), c->x_can, sizeof(float4) * n, hipMemcpyDeviceToHost,
-                           c->stream));
-    SPH_HIP(hipMemcpyAsync(hv.data(), c->v_can, sizeof(float4) * n, hipMemcpyDeviceToHost,
-                           c->stream));
+  float *dst[6] = {x, y, z, vx, vy, vz};
+  const bool with_v = vx || vy || vz;
+  if (n && (x || y || z || with_v)) {
+    if ((rc = io_buffers(c))) return rc;
+    const size_t un = (size_t)n;
+    k_unpack_state<<<blocks_for((uint32_t)n), kBlock, 0, c->stream>>>(
+        (uint32_t)n, c->x_can, c->v_can, c->io_dev, with_v ? 1 : 0);
+    SPH_HIP(hipGetLastError());
+    SPH_HIP(hipMemcpyAsync(c->io_host, c->io_dev, sizeof(float) * (with_v ? 6 : 3) * un,
+                           hipMemcpyDeviceToHost, c->stream));
+    SPH_HIP(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < 6; ++k)
+      if (dst[k]) memcpy(dst[k], c->io_host + k * un, sizeof(float) * un);
+    return SPH_OK;
   }
   SPH_HIP(hipStreamSynchronize(c->stream));
-  for (int i = 0; i < n; ++i) {
-    if (x) x[i] = hx[i].x;
-    if (y) y[i] = hx[i].y;
-    if (z) z[i] = hx[i].z;
-    if (vx) vx[i] = hv[i].x;
-    if (vy) vy[i] = hv[i].y;
-    if (vz) vz[i] = hv[i].z;
-  }
   return SPH_OK;
 }
 

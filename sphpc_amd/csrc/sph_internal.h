@@ -178,6 +178,9 @@ struct sph_ctx {
   float *rho = nullptr;
   uint32_t *list = nullptr;     // sorted neighbour indices, quads of slots per 64 particles
   uint4 *list_head = nullptr;   // per sorted particle: layer bases and counts
+  // state transfers: SoA x, y, z, vx, vy, vz of up to cap particles, on the
+  // device and in pinned host memory (allocated on first use)
+  float *io_dev = nullptr, *io_host = nullptr;
   uint32_t *stats = nullptr;  // sph::STAT_WORDS words
   uint32_t *pair_partial = nullptr;  // per-block list-length sums of the last search
   uint32_t pair_blocks = 0;

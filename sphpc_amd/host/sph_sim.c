@@ -184,7 +184,8 @@ int main(int argc, char **argv) {
       CHECK(sph_update_positions(ctx));
     }
     if (output && cfg.output_every >= 0 && every > 0 && step % every == 0) {
-      CHECK(sph_download_state(ctx, x, y, z, vx, vy, vz));
+      /* a frame holds positions only (fileio_MPI_IO.c:79-106) */
+      CHECK(sph_download_state(ctx, x, y, z, NULL, NULL, NULL));
       write_frame(frame++, n, x, y, z);
     }
   }
