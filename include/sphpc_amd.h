@@ -141,7 +141,9 @@ int sph_set_obstacle(sph_ctx *ctx, const sph_obstacle_desc *obstacle);
 /* Host -> device state (the acc enter data of particles.c:1056-1078). */
 int sph_upload_state(sph_ctx *ctx, int n, const float *x, const float *y, const float *z,
                      const float *vx, const float *vy, const float *vz);
-/* Device -> host state in particle-id order (acc update host, fileio_adios.c:73-76). */
+/* Device -> host state in particle-id order (acc update host, fileio_adios.c:73-76).
+ * NULL arrays are skipped (x, y, z only for a frame); the copy goes through
+ * pinned staging that the context allocates on its first transfer. */
 int sph_download_state(sph_ctx *ctx, float *x, float *y, float *z, float *vx, float *vy,
                        float *vz);
 int sph_particle_count(sph_ctx *ctx, int *n);
