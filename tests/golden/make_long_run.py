@@ -39,6 +39,13 @@ CASES = {
         workload="sphere_block", scale=1.0, iters=4, substeps=150, every=15,
         physics=dict(k=0.5, c=0.01, lambda_eps=600.0, vort_eps=0.01, vort_model=1,
                      surface=True, viscosity=True, vorticity=True)),
+    # the same block over the full 1000 substeps of its configuration: it falls,
+    # splits on the sphere and spreads over the floor (about 35 minutes of CPU on
+    # 8 threads)
+    "sphere_block_1m_1000": dict(
+        workload="sphere_block", scale=1.0, iters=4, substeps=1000, every=100,
+        physics=dict(k=0.5, c=0.01, lambda_eps=600.0, vort_eps=0.01, vort_model=1,
+                     surface=True, viscosity=True, vorticity=True)),
     # the full 8M double dam break: after about 80 substeps the 1 m water columns
     # compress past what 4 Jacobi iterations hold, particles pile past the
     # 60-neighbour cap and the maximum error grows to ~10 (the statistics of the
