@@ -37,8 +37,9 @@ BOUNDS = {"ke": (0.03, 1e-3), "mean_err": (0.10, 5e-4), "p99_err": (0.25, 2e-3),
 # splash off the sphere is chaotic.  From substep 300 on the kinetic energies of
 # the FP32 and FP64 runs differ by 3-20%; two FP32 runs on the GPU whose initial
 # states differ by one float32 ulp per coordinate differ by 3-18% over the same
-# checkpoints, so that is the flow's own sensitivity.  The density statistics
-# stay within the common bounds.
+# checkpoints, and two FP64 oracle runs with that perturbation by up to 50% (mean
+# error up to 15%, 99th percentile up to 42%), so that is the flow's own
+# sensitivity.  The density statistics stay within the common bounds.
 CASE_BOUNDS = {"double_dam_break_8m": {"ke": (0.08, 1e-3)},
                "sphere_block_1m_1000": {"ke": (0.25, 1e-3)}}
 
