@@ -202,8 +202,12 @@ int sph_download_sorted(sph_ctx *ctx, uint32_t *keys, uint32_t *ids);
 /* Cell tables: start[c] = lower bound, end[c] = upper bound (cells entries). */
 int sph_download_cell_bounds(sph_ctx *ctx, uint32_t *start, uint32_t *end);
 
-/* --- native sub-boundary (device pointers; stream NULL = default) ------- */
-/* SortByKey, thrust_c.h:47-50: stable sort of n (key, value) pairs by key. */
+/* --- native sub-boundary (device pointers; stream NULL = default) -------
+ * These keep their scratch per device between calls (allocated on the first
+ * call of a size, held until the process ends); calls on one device take
+ * turns, and each returns after its work on `stream` is done. */
+/* SortByKey, thrust_c.h:47-50: stable sort of n (key, value) pairs by key.
+ * The radix passes follow the keys' highest set bit (one OR-reduction first). */
 int sph_sort_by_key(uint32_t *keys, uint32_t *values, int n, void *stream);
 /* Same, sorting only the low `key_bits` bits (keys must be < 2^key_bits). */
 int sph_sort_by_key_bits(uint32_t *keys, uint32_t *values, int n, int key_bits,

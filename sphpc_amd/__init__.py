@@ -589,13 +589,17 @@ class DeviceBuffer:
             pass
 
 
-def sort_by_key(keys, values, key_bits=32):
-    """SortByKey (thrust_c.cu:86-91) on device copies; returns host arrays."""
+def sort_by_key(keys, values, key_bits=None):
+    """SortByKey (thrust_c.cu:86-91) on device copies; returns host arrays.
+    key_bits None: sph_sort_by_key (passes from the keys' highest set bit)."""
     keys = np.ascontiguousarray(keys, np.uint32)
     values = np.ascontiguousarray(values, np.uint32)
     n = len(keys)
     kb, vb = DeviceBuffer.from_array(keys), DeviceBuffer.from_array(values)
-    check(lib().sph_sort_by_key_bits(kb.ptr, vb.ptr, n, int(key_bits), None))
+    if key_bits is None:
+        check(lib().sph_sort_by_key(kb.ptr, vb.ptr, n, None))
+    else:
+        check(lib().sph_sort_by_key_bits(kb.ptr, vb.ptr, n, int(key_bits), None))
     return kb.to_array(np.uint32, n), vb.to_array(np.uint32, n)
 
 

@@ -10,6 +10,8 @@
 //  - FindLowerBounds/FindUpperBounds: a key histogram followed by an
 //    exclusive scan gives lower_bound(c) for every cell; upper_bound(c) is
 //    lower_bound(c+1).
+#include <mutex>
+
 #include "sph_internal.h"
 
 namespace sph {
@@ -392,45 +394,117 @@ using namespace sph;
 // --------------------------------------------------------------------------
 // C ABI: native sort / bounds (replacing thrust_c.cu:86-120)
 // --------------------------------------------------------------------------
-extern "C" int sph_sort_by_key_bits(uint32_t *keys, uint32_t *values, int n, int key_bits,
-                                    void *stream) {
-  if (n < 0 || (n > 0 && (!keys || !values)) || key_bits < 1 || key_bits > 32) {
+// Scratch of the native sort / bounds entry points, kept per device between
+// calls (grow-only, held until the process ends) so that a caller that sorts
+// every substep does not allocate each time.  Calls on one device take turns;
+// each ends with a synchronize of the caller's stream.
+namespace {
+struct AbiScratch {
+  std::mutex m;
+  uint32_t *a = nullptr, *b = nullptr;  // `words` each
+  size_t words = 0;
+  uint32_t *hist = nullptr;
+  size_t hist_cap = 0;
+  ScanScratch sc;
+  uint32_t *word = nullptr;  // one word for the key OR-reduction
+};
+constexpr int kAbiDevices = 64;
+AbiScratch g_abi[kAbiDevices];
+
+int abi_scratch(AbiScratch **out) {
+  int dev = 0;
+  SPH_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kAbiDevices) {
+    set_error("device index beyond the native entry points' scratch table");
+    return SPH_ERR_ARG;
+  }
+  *out = &g_abi[dev];
+  return SPH_OK;
+}
+
+// a and b hold at least `words` words each (contents not kept on growth)
+int abi_words(AbiScratch *w, size_t words) {
+  if (w->words >= words) return SPH_OK;
+  if (w->a) SPH_HIP(hipFree(w->a));
+  if (w->b) SPH_HIP(hipFree(w->b));
+  w->a = w->b = nullptr;
+  w->words = 0;
+  hipError_t e = hipMalloc(&w->a, sizeof(uint32_t) * words);
+  if (e == hipSuccess) e = hipMalloc(&w->b, sizeof(uint32_t) * words);
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc (native entry point scratch)");
+  w->words = words;
+  return SPH_OK;
+}
+}  // namespace
+
+// OR of all keys (one device-scope atomic per wave): the highest set bit gives
+// the radix passes a 32-bit sort needs.
+__global__ __launch_bounds__(kBlock) void k_or_keys(const uint32_t *__restrict__ keys,
+                                                    uint32_t n, uint32_t *__restrict__ out) {
+  uint32_t v = 0;
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+    v |= keys[i];
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v |= __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0 && v) atomicOr(out, v);
+}
+
+static int sort_impl(uint32_t *keys, uint32_t *values, int n, int key_bits, void *stream) {
+  if (n < 0 || (n > 0 && (!keys || !values)) || key_bits < 0 || key_bits > 32) {
     set_error("sph_sort_by_key: invalid argument");
     return SPH_ERR_ARG;
   }
   if (n <= 1) return SPH_OK;
   hipStream_t s = (hipStream_t)stream;
-  uint32_t *tk = nullptr, *tv = nullptr, *hist = nullptr;
-  size_t hist_cap = 0;
-  ScanScratch sc;
-  int rc = SPH_OK;
-  const uint32_t *kout = nullptr, *vout = nullptr;
-  hipError_t e = hipMalloc(&tk, sizeof(uint32_t) * (size_t)n);
-  if (e == hipSuccess) e = hipMalloc(&tv, sizeof(uint32_t) * (size_t)n);
-  if (e != hipSuccess) {
-    rc = hip_fail(e, "hipMalloc (sort scratch)");
-  } else {
-    rc = radix_sort_pairs(keys, values, tk, tv, keys, values, (uint32_t)n, key_bits, &hist,
-                          &hist_cap, &sc, s, nullptr, &kout, &vout);
-    if (rc == SPH_OK && kout != keys) {
-      e = hipMemcpyAsync(keys, kout, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToDevice, s);
-      if (e == hipSuccess)
-        e = hipMemcpyAsync(values, vout, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToDevice,
-                           s);
-      if (e != hipSuccess) rc = hip_fail(e, "hipMemcpyAsync (sort result)");
+  AbiScratch *w = nullptr;
+  int rc = abi_scratch(&w);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lock(w->m);
+  if ((rc = abi_words(w, (size_t)n))) return rc;
+  if (key_bits == 0) {  // unknown key range: from the keys' OR
+    if (!w->word) {
+      hipError_t e = hipMalloc(&w->word, sizeof(uint32_t));
+      if (e != hipSuccess) {
+        w->word = nullptr;
+        return hip_fail(e, "hipMalloc (key range)");
+      }
     }
-    e = hipStreamSynchronize(s);
-    if (rc == SPH_OK && e != hipSuccess) rc = hip_fail(e, "hipStreamSynchronize (sort)");
+    k_fill_u32<<<1, 64, 0, s>>>(w->word, 0u, 1u);
+    k_or_keys<<<std::min(blocks_for((uint32_t)n), 1024u), kBlock, 0, s>>>(keys, (uint32_t)n,
+                                                                          w->word);
+    SPH_HIP(hipGetLastError());
+    uint32_t all = 0;
+    SPH_HIP(hipMemcpyAsync(&all, w->word, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    SPH_HIP(hipStreamSynchronize(s));
+    key_bits = all ? 32 - __builtin_clz(all) : 1;
   }
-  if (tk) (void)hipFree(tk);
-  if (tv) (void)hipFree(tv);
-  if (hist) (void)hipFree(hist);
-  if (sc.tile_sums) (void)hipFree(sc.tile_sums);
+  const uint32_t *kout = nullptr, *vout = nullptr;
+  rc = radix_sort_pairs(keys, values, w->a, w->b, keys, values, (uint32_t)n, key_bits, &w->hist,
+                        &w->hist_cap, &w->sc, s, nullptr, &kout, &vout);
+  if (rc == SPH_OK && kout != keys) {
+    hipError_t e =
+        hipMemcpyAsync(keys, kout, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(values, vout, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) rc = hip_fail(e, "hipMemcpyAsync (sort result)");
+  }
+  const hipError_t e = hipStreamSynchronize(s);
+  if (rc == SPH_OK && e != hipSuccess) rc = hip_fail(e, "hipStreamSynchronize (sort)");
   return rc;
 }
 
+extern "C" int sph_sort_by_key_bits(uint32_t *keys, uint32_t *values, int n, int key_bits,
+                                    void *stream) {
+  if (key_bits < 1) {
+    set_error("sph_sort_by_key: invalid argument");
+    return SPH_ERR_ARG;
+  }
+  return sort_impl(keys, values, n, key_bits, stream);
+}
+
+// Keys of any range: the passes follow the keys' highest set bit.
 extern "C" int sph_sort_by_key(uint32_t *keys, uint32_t *values, int n, void *stream) {
-  return sph_sort_by_key_bits(keys, values, n, 32, stream);
+  return sort_impl(keys, values, n, 0, stream);
 }
 
 static int bounds_impl(const uint32_t *keys, int n, int cells, uint32_t *out, void *stream,
@@ -441,28 +515,23 @@ static int bounds_impl(const uint32_t *keys, int n, int cells, uint32_t *out, vo
   }
   if (cells == 0) return SPH_OK;
   hipStream_t s = (hipStream_t)stream;
-  uint32_t *count = nullptr, *start = nullptr;
-  ScanScratch sc;
-  int rc = SPH_OK;
-  hipError_t e = hipMalloc(&count, sizeof(uint32_t) * ((size_t)cells + 1));
-  if (e == hipSuccess) e = hipMalloc(&start, sizeof(uint32_t) * ((size_t)cells + 1));
-  if (e != hipSuccess) {
-    rc = hip_fail(e, "hipMalloc (bounds scratch)");
-  } else {
-    rc = count_keys(keys, (uint32_t)n, count, (uint32_t)cells, s);
-    if (rc == SPH_OK)
-      rc = exclusive_scan(count, start, (uint32_t)cells, false, true, &sc, s, nullptr);
-    if (rc == SPH_OK) {
-      e = hipMemcpyAsync(out, start + (upper ? 1 : 0), sizeof(uint32_t) * (size_t)cells,
-                         hipMemcpyDeviceToDevice, s);
-      if (e != hipSuccess) rc = hip_fail(e, "hipMemcpyAsync (bounds)");
-    }
-    e = hipStreamSynchronize(s);
-    if (rc == SPH_OK && e != hipSuccess) rc = hip_fail(e, "hipStreamSynchronize (bounds)");
+  AbiScratch *w = nullptr;
+  int rc = abi_scratch(&w);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lock(w->m);
+  if ((rc = abi_words(w, (size_t)cells + 1))) return rc;
+  uint32_t *count = w->a, *start = w->b;
+  rc = count_keys(keys, (uint32_t)n, count, (uint32_t)cells, s);
+  if (rc == SPH_OK)
+    rc = exclusive_scan(count, start, (uint32_t)cells, false, true, &w->sc, s, nullptr);
+  if (rc == SPH_OK) {
+    const hipError_t e = hipMemcpyAsync(out, start + (upper ? 1 : 0),
+                                        sizeof(uint32_t) * (size_t)cells,
+                                        hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) rc = hip_fail(e, "hipMemcpyAsync (bounds)");
   }
-  if (count) (void)hipFree(count);
-  if (start) (void)hipFree(start);
-  if (sc.tile_sums) (void)hipFree(sc.tile_sums);
+  const hipError_t e = hipStreamSynchronize(s);
+  if (rc == SPH_OK && e != hipSuccess) rc = hip_fail(e, "hipStreamSynchronize (bounds)");
   return rc;
 }
 

@@ -39,14 +39,17 @@ def _increment_err(v_before, v_after_gpu, v_after_64):
 # ---------------------------------------------------------------------------
 # Native sub-boundary: sort and bounds (thrust_c.cu:86-120)
 # ---------------------------------------------------------------------------
+@pytest.mark.parametrize("adaptive", [False, True])
 @pytest.mark.parametrize("n,bits,hi", [(0, 8, 1), (1, 8, 200), (4095, 16, 7),
                                        (4097, 23, 2 ** 23), (100_000, 23, 5000),
-                                       (1_000_003, 32, 2 ** 32)])
-def test_sort_by_key_bit_exact(n, bits, hi):
+                                       (1_000_003, 32, 2 ** 32), (5000, 8, 1)])
+def test_sort_by_key_bit_exact(n, bits, hi, adaptive):
+    """With the key width given, and with it taken from the keys' OR (all
+    keys zero in the last case)."""
     rng = np.random.default_rng(n)
     keys = rng.integers(0, min(hi, 2 ** bits), n, dtype=np.uint64).astype(np.uint32)
     vals = rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32)
-    k, v = sa.sort_by_key(keys, vals, key_bits=bits)
+    k, v = sa.sort_by_key(keys, vals, key_bits=None if adaptive else bits)
     order = np.argsort(keys, kind="stable")
     assert np.array_equal(k, keys[order])
     assert np.array_equal(v, vals[order])
