@@ -191,6 +191,7 @@ static void free_all(sph_ctx *c) {
 
 static int require_state(sph_ctx *c) {
   if (!c) return arg_error("context is NULL");
+  c->gen++;
   if (!c->state_valid) return state_error("no particle state uploaded");
   return SPH_OK;
 }
@@ -472,6 +473,7 @@ extern "C" int sph_batch_scene_errors(sph_ctx *c, float *max_err) {
 }
 
 extern "C" int sph_set_params(sph_ctx *c, const sph_params *p) {
+  if (c) c->gen++;
   if (!c) return arg_error("context is NULL");
   int rc = check_params(p);
   if (rc) return rc;
@@ -532,6 +534,7 @@ static int make_sdf(const sph_obstacle_desc *o, SdfDev *d, float **buf, hipStrea
 }
 
 extern "C" int sph_set_obstacle(sph_ctx *c, const sph_obstacle_desc *o) {
+  if (c) c->gen++;
   if (!c) return arg_error("context is NULL");
   if (c->nscenes) return state_error("a batched context takes its obstacles from sph_create_batch");
   graph_reset(c);
@@ -601,6 +604,7 @@ static int io_buffers(sph_ctx *c) {
 extern "C" int sph_upload_state(sph_ctx *c, int n, const float *x, const float *y,
                                 const float *z, const float *vx, const float *vy,
                                 const float *vz) {
+  if (c) c->gen++;
   if (!c) return arg_error("context is NULL");
   if (n < 0) return arg_error("n must be >= 0");
   if (n > c->cap) {
@@ -692,13 +696,22 @@ extern "C" int sph_compute_densities(sph_ctx *c) {
   const int slot = std::min(c->staged_slot, kMaxIters);
   c->staged_slot++;
   c->last_err_slot = slot;
-  // density_to_w keeps rho in pos.w for the velocity passes
-  return launch_density(c, 3, slot);
+  // density_to_w keeps rho in pos.w for the velocity passes.  With a fixed
+  // iteration count the same gather also computes lambda and keeps it aside
+  // for an immediately following sph_compute_lambda.
+  if (c->P.density_tolerance > 0.0) return launch_density(c, 3, slot);
+  if ((rc = launch_density(c, 4, slot))) return rc;
+  c->lam_gen = c->gen;
+  return SPH_OK;
 }
 
 extern "C" int sph_compute_lambda(sph_ctx *c) {
+  // lambda from the density pass just before, when no call came in between
+  // (the same values: one gather computes both from the same positions)
+  const bool fresh = c && c->lam_gen == c->gen;
   int rc = require_lists(c);
-  return rc ? rc : launch_density(c, 2, -1);
+  if (rc) return rc;
+  return fresh ? launch_lambda_from_density(c) : launch_density(c, 2, -1);
 }
 
 extern "C" int sph_update_dps(sph_ctx *c, int substep) {
@@ -1190,6 +1203,7 @@ static const int kTunable[] = {K_BUILD_LISTS, K_DENSITY_LAMBDA, K_DENSITY, K_LAM
                                K_CONFINE, K_WARM_LAMBDA};
 
 extern "C" int sph_autotune(sph_ctx *c, int substeps, sph_tuning *out) {
+  if (c) c->gen++;
   int rc = require_state(c);
   if (rc) return rc;
   if (substeps < 1) return arg_error("substeps must be >= 1");

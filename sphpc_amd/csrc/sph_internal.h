@@ -223,6 +223,10 @@ struct sph_ctx {
   } graph_after{};
   int last_err_slot = -1;
   int staged_slot = 0;
+  // state generation: bumped by every call that may change the state; the
+  // stage path's density pass records it, and the lambda stage reuses that
+  // pass's lambda when nothing ran in between
+  unsigned long long gen = 0, lam_gen = ~0ull;
   bool state_valid = false;
   bool sorted_valid = false;
   bool lists_valid = false;
@@ -262,7 +266,11 @@ int count_keys(const uint32_t *keys, uint32_t n, uint32_t *count, uint32_t cells
 int launch_gravity(sph_ctx *c);
 int launch_predict(sph_ctx *c);
 int launch_neighbor_search(sph_ctx *c, bool fused);
+// mode 0: density + lambda (fused substep), 1: density, 2: lambda, 3: density
+// with rho into pos.w (tolerance loop), 4: density + lambda with rho into pos.w
+// and lambda into dpbuf (the stage path; launch_lambda_from_density moves it)
 int launch_density(sph_ctx *c, int mode, int err_slot);
+int launch_lambda_from_density(sph_ctx *c);
 int launch_dp(sph_ctx *c, int substep);
 int launch_position_stars(sph_ctx *c);
 int launch_dp_update(sph_ctx *c, int substep, int err_slot, bool accumulate = false);

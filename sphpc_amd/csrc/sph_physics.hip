@@ -584,7 +584,7 @@ template <int MODE, int NB>
 __global__ __launch_bounds__(kBlock) void k_density_lambda(
     uint32_t n, uint32_t nblocks, float4 *pos, const uint32_t *__restrict__ list,
     const uint4 *__restrict__ head, DevParams P, float *rho_io, uint32_t *stats, int err_slot,
-    int check_converged, int density_to_w) {
+    int check_converged, int density_to_w, float *__restrict__ lam_out) {
   if (check_converged && converged(stats)) return;
   const uint32_t s = particle_index(nblocks);
   const bool active = s < n;
@@ -626,7 +626,10 @@ __global__ __launch_bounds__(kBlock) void k_density_lambda(
       const float Ci = fmaxf((rho - P.rest_density) * P.inv_rest_density, 0.0f);
       const float sum_c = P.lam_k2 * (sc + sq3(sgx, sgy, sgz));
       const float lam = -Ci / (sum_c + P.lambda_eps);
-      pos[s].w = lam;
+      if (lam_out)
+        lam_out[s] = lam;  // pos.w keeps rho (density_to_w)
+      else
+        pos[s].w = lam;
     }
   }
   if (MODE != 2 && err_slot >= 0) wave_max_to(&stats[STAT_ERR0 + err_slot], err, active);
@@ -1466,9 +1469,9 @@ int launch_density(sph_ctx *c, int mode, int err_slot) {
     with_depth(tuned_depth(c, K_DENSITY_LAMBDA), [&](auto D_) {
       constexpr int NBv = decltype(D_)::value;
       LAUNCH_TIMED(c, K_DENSITY_LAMBDA, 0,
-                   k_density_lambda<0, NBv><<<nb, bs, 0, s>>>(n, nb, c->pos, c->list, c->list_head,
-                                                             c->dev, c->rho, c->stats, err_slot,
-                                                             c->P.density_tolerance > 0, 0));
+                   k_density_lambda<0, NBv><<<nb, bs, 0, s>>>(
+                       n, nb, c->pos, c->list, c->list_head, c->dev, c->rho, c->stats, err_slot,
+                       c->P.density_tolerance > 0, 0, nullptr));
     });
   else if (mode == 1)
     with_depth(tuned_depth(c, K_DENSITY), [&](auto D_) {
@@ -1476,7 +1479,15 @@ int launch_density(sph_ctx *c, int mode, int err_slot) {
       LAUNCH_TIMED(c, K_DENSITY, 0,
                    k_density_lambda<1, NBv><<<nb, bs, 0, s>>>(n, nb, c->pos, c->list, c->list_head,
                                                              c->dev, c->rho, c->stats, err_slot, 0,
-                                                             0));
+                                                             0, nullptr));
+    });
+  else if (mode == 4)  // density and lambda in one gather: rho into w, lambda aside
+    with_depth(tuned_depth(c, K_DENSITY_LAMBDA), [&](auto D_) {
+      constexpr int NBv = decltype(D_)::value;
+      LAUNCH_TIMED(c, K_DENSITY_LAMBDA, 0,
+                   k_density_lambda<0, NBv><<<nb, bs, 0, s>>>(
+                       n, nb, c->pos, c->list, c->list_head, c->dev, c->rho, c->stats,
+                       err_slot, 0, 1, reinterpret_cast<float *>(c->dpbuf)));
     });
   else if (mode == 3)  // final density of a capped tolerance loop: rho also into w
     with_depth(tuned_depth(c, K_DENSITY), [&](auto D_) {
@@ -1484,15 +1495,33 @@ int launch_density(sph_ctx *c, int mode, int err_slot) {
       LAUNCH_TIMED(c, K_DENSITY, 0,
                    k_density_lambda<1, NBv><<<nb, bs, 0, s>>>(n, nb, c->pos, c->list, c->list_head,
                                                              c->dev, c->rho, c->stats, err_slot, 1,
-                                                             1));
+                                                             1, nullptr));
     });
   else
     with_depth(tuned_depth(c, K_LAMBDA), [&](auto D_) {
       constexpr int NBv = decltype(D_)::value;
       LAUNCH_TIMED(c, K_LAMBDA, 0,
                    k_density_lambda<2, NBv><<<nb, bs, 0, s>>>(n, nb, c->pos, c->list, c->list_head,
-                                                             c->dev, c->rho, c->stats, -1, 0, 0));
+                                                             c->dev, c->rho, c->stats, -1, 0, 0,
+                                                             nullptr));
     });
+  SPH_HIP(hipGetLastError());
+  return SPH_OK;
+}
+
+// pos.w = the lambda a mode-4 density pass put aside.
+__global__ __launch_bounds__(kBlock) void k_lambda_to_w(uint32_t n, const float *__restrict__ lam,
+                                                        float4 *__restrict__ pos) {
+  const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+  if (s < n) pos[s].w = lam[s];
+}
+
+int launch_lambda_from_density(sph_ctx *c) {
+  const uint32_t n = c->n;
+  if (!n) return SPH_OK;
+  LAUNCH_TIMED(c, K_LAMBDA, 0,
+               k_lambda_to_w<<<blocks_for(n), kBlock, 0, c->stream>>>(
+                   n, reinterpret_cast<const float *>(c->dpbuf), c->pos));
   SPH_HIP(hipGetLastError());
   return SPH_OK;
 }
