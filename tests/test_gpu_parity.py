@@ -462,3 +462,25 @@ def test_collision_entry_point_matches_oracle():
     # relative error ~1e-6 of the push distance, plus the FP32 rounding of x.
     push = float(np.max(np.linalg.norm(ref - d64(pts), axis=1)))
     assert e <= 1e-5 * push + 4 * np.spacing(np.float32(1.0)), (e, push)
+
+
+def test_stage_lambda_reuse_matches_a_fresh_gather():
+    """sph_compute_lambda right after sph_compute_densities takes the lambda
+    that the density gather computed; with a call in between (here a download of
+    the densities) it gathers again.  Both give the same bits, and so does the
+    position update that follows."""
+    sc, pos, vel = _scene_dam_block(spread=0.8, seed=3)
+    runs = []
+    for interpose in (False, True):
+        ctx = _staged_search(sc, pos, vel)
+        ctx.compute_densities()
+        rho = ctx.field(sa.FIELD_DENSITY) if interpose else None
+        ctx.compute_lambda()
+        lam = ctx.field(sa.FIELD_LAMBDA)
+        ctx.update_dps(0)
+        ctx.update_position_stars()
+        runs.append((lam, ctx.field(sa.FIELD_POSITION_STAR), rho))
+        ctx.close()
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert np.array_equal(runs[0][1], runs[1][1])
+    assert np.all(np.isfinite(runs[1][2])) and runs[1][2].min() > 0.0
