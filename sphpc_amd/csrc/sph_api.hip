@@ -718,12 +718,25 @@ extern "C" int sph_update_dps(sph_ctx *c, int substep) {
   int rc = require_lists(c);
   if (rc) return rc;
   if (substep < 0) return arg_error("substep must be >= 0");
-  return launch_dp(c, substep);
+  // the same pass also writes x* + delta p (clamped, pushed out of the
+  // obstacle) into the other position buffer, for an immediately following
+  // sph_update_position_stars; x* itself is unchanged until then
+  float4 *next = (c->pos == c->pos_a) ? c->pos_b : c->pos_a;
+  if ((rc = launch_dp(c, substep, next))) return rc;
+  c->dp_next = next;
+  c->dp_gen = c->gen;
+  return SPH_OK;
 }
 
 extern "C" int sph_update_position_stars(sph_ctx *c) {
+  const bool fresh = c && c->dp_gen == c->gen;
   int rc = require_lists(c);
-  return rc ? rc : launch_position_stars(c);
+  if (rc) return rc;
+  if (fresh) {
+    c->pos = c->dp_next;
+    return SPH_OK;
+  }
+  return launch_position_stars(c);
 }
 
 extern "C" int sph_update_velocities(sph_ctx *c) {

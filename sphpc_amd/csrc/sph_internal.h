@@ -227,6 +227,8 @@ struct sph_ctx {
   // stage path's density pass records it, and the lambda stage reuses that
   // pass's lambda when nothing ran in between
   unsigned long long gen = 0, lam_gen = ~0ull;
+  unsigned long long dp_gen = ~0ull;  // likewise the delta-p stage's updated x*
+  float4 *dp_next = nullptr;
   bool state_valid = false;
   bool sorted_valid = false;
   bool lists_valid = false;
@@ -271,7 +273,8 @@ int launch_neighbor_search(sph_ctx *c, bool fused);
 // and lambda into dpbuf (the stage path; launch_lambda_from_density moves it)
 int launch_density(sph_ctx *c, int mode, int err_slot);
 int launch_lambda_from_density(sph_ctx *c);
-int launch_dp(sph_ctx *c, int substep);
+// xnext: also the position update of launch_position_stars, into xnext
+int launch_dp(sph_ctx *c, int substep, float4 *xnext = nullptr);
 int launch_position_stars(sph_ctx *c);
 int launch_dp_update(sph_ctx *c, int substep, int err_slot, bool accumulate = false);
 int launch_velocity(sph_ctx *c);

@@ -704,17 +704,29 @@ __device__ __forceinline__ float3 dp_sum(const float4 &p, const float4 *pos, con
 }
 
 // Staged UpdateDPs: dp = stiffness * m * DelW_norm / rho0 * sum.
-template <int NB>
+// With xnext, also the position update of k_position_stars from the same dp
+// (x* + dp, clamp, collision, rho into w), into another buffer.
+template <bool BATCH, int NB>
 __global__ __launch_bounds__(kBlock) void k_dp(uint32_t n, uint32_t nblocks,
                                                const float4 *__restrict__ pos,
                                                const uint32_t *__restrict__ list,
                                                const uint4 *__restrict__ head, DevParams P,
-                                               float scale, float4 *__restrict__ dp) {
+                                               float scale, float4 *__restrict__ dp,
+                                               float4 *__restrict__ xnext,
+                                               const float *__restrict__ rho, SdfDev o,
+                                               BatchDev B) {
   const uint32_t s = particle_index(nblocks);
   if (s >= n) return;
   const float4 p = pos[s];
   const float3 a = dp_sum<NB>(p, pos, nb_list(list, head, s), s, P);
-  dp[s] = make_float4(scale * a.x, scale * a.y, scale * a.z, 0.0f);
+  const float4 d = make_float4(scale * a.x, scale * a.y, scale * a.z, 0.0f);
+  dp[s] = d;
+  if (xnext) {
+    float x = p.x + d.x, y = p.y + d.y, z = p.z + d.z;
+    apply_boundary(x, y, z, geo_of<BATCH>(s, P, B), P);
+    collide(x, y, z, sdf_of<BATCH>(s, o, B));
+    xnext[s] = make_float4(x, y, z, rho[s]);
+  }
 }
 
 // Staged UpdatePositionStars (particles.c:529-564): x* += dp, clamp, collide.
@@ -1526,16 +1538,26 @@ int launch_lambda_from_density(sph_ctx *c) {
   return SPH_OK;
 }
 
-int launch_dp(sph_ctx *c, int substep) {
+int launch_dp(sph_ctx *c, int substep, float4 *xnext) {
   const uint32_t n = c->n;
   if (!n) return SPH_OK;
   const uint32_t bs = tuned_block(c, K_DP);
   const uint32_t nb = blocks_for(n, bs);
+  const float scale = stiffness_scale(c, substep);
+  const BatchDev B = c->batch();
   with_depth(tuned_depth(c, K_DP), [&](auto D_) {
     constexpr int NBv = decltype(D_)::value;
-    LAUNCH_TIMED(c, K_DP, 0,
-               k_dp<NBv><<<nb, bs, 0, c->stream>>>(n, nb, c->pos, c->list, c->list_head, c->dev,
-                                                  stiffness_scale(c, substep), c->dpbuf));
+    if (c->nscenes)
+      LAUNCH_TIMED(c, K_DP, 0,
+                   (k_dp<true, NBv><<<nb, bs, 0, c->stream>>>(n, nb, c->pos, c->list,
+                                                             c->list_head, c->dev, scale,
+                                                             c->dpbuf, xnext, c->rho, c->sdf, B)));
+    else
+      LAUNCH_TIMED(c, K_DP, 0,
+                   (k_dp<false, NBv><<<nb, bs, 0, c->stream>>>(n, nb, c->pos, c->list,
+                                                              c->list_head, c->dev, scale,
+                                                              c->dpbuf, xnext, c->rho, c->sdf,
+                                                              B)));
   });
   SPH_HIP(hipGetLastError());
   return SPH_OK;
