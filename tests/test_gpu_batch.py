@@ -87,6 +87,52 @@ def test_batch_bit_identical_to_separate_contexts(graph):
            graph=int(graph))
 
 
+def _stage_substep(ctx, iters):
+    """One substep through the stage entry points, in the reference's order."""
+    ctx.apply_gravity()
+    ctx.predict_positions()
+    ctx.find_all_neighbors()
+    for it in range(iters):
+        ctx.compute_densities()
+        ctx.compute_lambda()
+        ctx.update_dps(it)
+        ctx.update_position_stars()
+    ctx.update_velocities()
+    ctx.compute_surface_tension()
+    ctx.apply_viscosity()
+    ctx.compute_vorticity()
+    ctx.apply_vorticity_confinement()
+    ctx.update_positions()
+
+
+def test_batch_stage_path_bit_identical_to_separate_contexts():
+    """The stage entry points on a batched context (per-scene boxes and
+    obstacles in the delta-p stage) give the bits of separate contexts."""
+    s, scenes = _scenes(11, 4)
+    p = _params(s)
+    batch = sa.BatchContext(p, scenes)
+    batch.upload(np.concatenate([sc["pos"] for sc in scenes]),
+                 np.concatenate([sc["vel"] for sc in scenes]))
+    singles = []
+    for sc in scenes:
+        c = sa.Context(p, sc["size"], sc["particles"])
+        if sc["sdf"] is not None:
+            o = sc["sdf"]
+            c.set_obstacle(o["sdf"], o["dims"], o["spacing"], o["wmin"], o["wmax"])
+        c.upload(sc["pos"], sc["vel"])
+        singles.append(c)
+    for step in range(3):
+        _stage_substep(batch, p.solve_iterations)
+        for c in singles:
+            _stage_substep(c, p.solve_iterations)
+        bp, bv = batch.download()
+        for k, c in enumerate(singles):
+            a, b = batch.first[k], batch.first[k + 1]
+            sp, sv = c.download()
+            assert np.array_equal(bp[a:b], sp), (step, k)
+            assert np.array_equal(bv[a:b], sv), (step, k)
+
+
 def test_batch_rejects_partial_upload_and_obstacle_change():
     s, scenes = _scenes(3, 2)
     batch = sa.BatchContext(_params(s), scenes)
