@@ -225,6 +225,14 @@ int sph_sdf_create(const sph_obstacle_desc *desc, sph_sdf **out);
 int sph_sdf_destroy(sph_sdf *sdf);
 int sph_calculate_particle_collisions(float *x, float *y, float *z, int n, const sph_sdf *sdf,
                                       void *stream);
+/* The same on the reference's own FP64 device arrays, as CalculateParticle-
+ * Collisions (obstacle_cuda.h:25-29, obstacle_cuda.cu:56-116) takes them: the
+ * bounds test and normalised coordinates in double, the SDF arithmetic in
+ * FP32, the push added to the double coordinates.  The reference's by-value
+ * struct Obstacle (world box, SDF dims and spacing, a CUDA texture) is the
+ * sph_sdf made from the same fields by sph_sdf_create. */
+int sph_calculate_particle_collisions_f64(double *x, double *y, double *z, int n,
+                                          const sph_sdf *sdf, void *stream);
 
 /* Device memory plumbing for callers that hold raw device pointers. */
 int sph_device_malloc(void **ptr, size_t bytes);

@@ -119,6 +119,7 @@ def lib():
     L.sph_sdf_create.argtypes = [C.POINTER(ObstacleDesc), C.POINTER(vp)]
     L.sph_sdf_destroy.argtypes = [vp]
     L.sph_calculate_particle_collisions.argtypes = [vp, vp, vp, ip, vp, vp]
+    L.sph_calculate_particle_collisions_f64.argtypes = [vp, vp, vp, ip, vp, vp]
     L.sph_device_malloc.argtypes = [C.POINTER(vp), C.c_size_t]
     L.sph_device_free.argtypes = [vp]
     L.sph_device_copy.argtypes = [vp, vp, C.c_size_t, vp]
@@ -671,18 +672,23 @@ def sdf_sample_shapes(shapes, dims, wmin, wmax):
 
 
 def collide(pos, sdf, dims, spacing, wmin, wmax):
-    """CalculateParticleCollisions (obstacle_cuda.cu:118-131) on SoA floats."""
-    pos = _f32(pos).reshape(-1, 3)
+    """CalculateParticleCollisions (obstacle_cuda.cu:118-131) on device SoA
+    arrays: float32 positions, or float64 ones (the reference's own arrays,
+    sph_calculate_particle_collisions_f64).  Returns host positions of the
+    same type."""
+    dt = np.float64 if np.asarray(pos).dtype == np.float64 else np.float32
+    pos = np.ascontiguousarray(pos, dt).reshape(-1, 3)
     n = len(pos)
     sdf = _f32(sdf).ravel()
     d = ObstacleDesc(int(dims[0]), int(dims[1]), int(dims[2]), float(spacing),
                      AABB(wmin[0], wmax[0], wmin[1], wmax[1], wmin[2], wmax[2]), sdf.ctypes.data)
     h = C.c_void_p()
     check(lib().sph_sdf_create(C.byref(d), C.byref(h)))
+    entry = (lib().sph_calculate_particle_collisions_f64 if dt == np.float64 else
+             lib().sph_calculate_particle_collisions)
     try:
         bufs = [DeviceBuffer.from_array(np.ascontiguousarray(pos[:, i])) for i in range(3)]
-        check(lib().sph_calculate_particle_collisions(bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, n,
-                                                      h, None))
-        return np.stack([b.to_array(np.float32, n) for b in bufs], 1)
+        check(entry(bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, n, h, None))
+        return np.stack([b.to_array(dt, n) for b in bufs], 1)
     finally:
         lib().sph_sdf_destroy(h)

@@ -89,7 +89,7 @@ static const char *kKernelNames[K_COUNT] = {
     "predict_hash", "hash", "gravity", "predict", "radix_upsweep", "radix_downsweep",
     "scan", "reorder", "build_lists", "density_lambda", "density", "lambda", "dp",
     "dp_update", "position_stars", "velocity", "color", "surface", "xsph", "vorticity",
-    "confine", "commit", "aux", "warm_lambda", "collide"};
+    "confine", "commit", "aux", "warm_lambda", "collide", "lambda_copy"};
 
 // Derived FP32 constants (SetParameters, simulation.c:147-177; the constants
 // of particles.c named in DevParams).  Computed in double, rounded once.
@@ -119,6 +119,17 @@ static void derive(sph_ctx *c) {
   d.inv_rest_density = (float)(1.0 / rho0);
   d.mW0 = (float)(m * W_norm * pow(h, 6.0));
   d.mWn = (float)(m * W_norm);
+  {
+    const double h2f = (double)d.h2;  // h is a float, so h * h is exact in double
+    d.h2_err = (float)(h2f - h * h);
+    d.dens_k = (float)(ldexp(1.0, 26) / (h2f * h2f * h2f));
+    const double a = m * W_norm / (rho0 * (double)d.dens_k);
+    const double c0 = m * W_norm * pow(h, 6.0) / rho0 - 1.0;
+    d.dens_a_hi = (float)a;
+    d.dens_a_lo = (float)(a - (double)d.dens_a_hi);
+    d.dens_c0_hi = (float)c0;
+    d.dens_c0_lo = (float)(c0 - (double)d.dens_c0_hi);
+  }
   d.lam_k2 = (float)((m * DelW_norm / rho0) * (m * DelW_norm / rho0));
   d.lambda_eps = (float)P.lambda_epsilon;
   d.dt = (float)P.time_step;
@@ -172,7 +183,8 @@ static void free_all(sph_ctx *c) {
   graph_reset(c);
   void *bufs[] = {c->x_can, c->v_can, c->xs_can, c->keys, c->keys_alt, c->ids, c->ids_alt,
                   c->hist, c->cell_end, c->cell_start, c->pos_a, c->pos_b, c->xold,
-                  c->vel_a, c->vel_b, c->aux, c->dpbuf, c->rho, c->list, c->list_head,
+                  c->vel_a, c->vel_b, c->aux, c->dpbuf, c->rho, c->cstage, c->lam_stage,
+                  c->list, c->list_head,
                   c->stats, c->pair_partial, c->red, c->sdf_buf, c->scan.tile_sums,
                   c->pw, c->lam_acc, c->lam_prev, c->feat, c->scenes_dev, c->scene_of,
                   c->batch_sdf, c->scene_err, c->scene_err0,
@@ -278,6 +290,8 @@ static int ctx_allocate(sph_ctx *c, int max_particles, sph_ctx **out) {
       {(void **)&c->xold, f4},        {(void **)&c->vel_a, f4},
       {(void **)&c->vel_b, f4},       {(void **)&c->aux, f4},
       {(void **)&c->dpbuf, f4},       {(void **)&c->rho, sizeof(float) * cap},
+      {(void **)&c->cstage, sizeof(float) * cap},
+      {(void **)&c->lam_stage, sizeof(float) * cap},
       {(void **)&c->list, sizeof(uint32_t) * groups * kListStride},
       {(void **)&c->list_head, sizeof(uint4) * cap},
       {(void **)&c->pw, f4},
@@ -292,6 +306,9 @@ static int ctx_allocate(sph_ctx *c, int max_particles, sph_ctx **out) {
   e = hipMemsetAsync(c->cell_end, 0, sizeof(uint32_t) * (c->cells + 1ull), c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(c->stats, 0, sizeof(uint32_t) * STAT_WORDS, c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(c->list_head, 0, sizeof(uint4) * cap, c->stream);
+  // C = 0 until a density stage runs (lambda from it is zero, as the
+  // reference's from its zero-initialised density array)
+  if (e == hipSuccess) e = hipMemsetAsync(c->cstage, 0, sizeof(float) * cap, c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(c->pos_a, 0, f4 + sizeof(float4) * kPosPad, c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(c->pos_b, 0, f4 + sizeof(float4) * kPosPad, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -1147,6 +1164,7 @@ extern "C" int sph_download_cell_bounds(sph_ctx *c, uint32_t *start, uint32_t *e
 // --------------------------------------------------------------------------
 struct sph_sdf {
   SdfDev dev;
+  SdfBox64 box;  // the world box as given (double)
   float *buf;
 };
 
@@ -1155,6 +1173,8 @@ extern "C" int sph_sdf_create(const sph_obstacle_desc *desc, sph_sdf **out) {
   *out = nullptr;
   sph_sdf *s = new sph_sdf();
   s->buf = nullptr;
+  const sph_aabb &wb = desc->world_bounds;
+  s->box = SdfBox64{{wb.min_x, wb.min_y, wb.min_z}, {wb.max_x, wb.max_y, wb.max_z}};
   int rc = make_sdf(desc, &s->dev, &s->buf, nullptr);
   if (rc) {
     if (s->buf) (void)hipFree(s->buf);
@@ -1176,6 +1196,15 @@ extern "C" int sph_calculate_particle_collisions(float *x, float *y, float *z, i
                                                  const sph_sdf *sdf, void *stream) {
   if (n < 0 || !sdf || (n > 0 && (!x || !y || !z))) return arg_error("invalid argument");
   int rc = launch_collide_soa(x, y, z, (uint32_t)n, sdf->dev, (hipStream_t)stream);
+  if (rc) return rc;
+  SPH_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return SPH_OK;
+}
+
+extern "C" int sph_calculate_particle_collisions_f64(double *x, double *y, double *z, int n,
+                                                     const sph_sdf *sdf, void *stream) {
+  if (n < 0 || !sdf || (n > 0 && (!x || !y || !z))) return arg_error("invalid argument");
+  int rc = launch_collide_soa_f64(x, y, z, (uint32_t)n, sdf->dev, sdf->box, (hipStream_t)stream);
   if (rc) return rc;
   SPH_HIP(hipStreamSynchronize((hipStream_t)stream));
   return SPH_OK;

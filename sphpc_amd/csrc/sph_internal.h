@@ -43,6 +43,14 @@ struct DevParams {
   float mass, rest_density, inv_rest_density;
   float mW0;                 // m * W(0)
   float mWn;                 // m * W_norm
+  // Density sum in 32-bit fixed point: each neighbour adds the truncated
+  // integer (h^2 - r^2)^3 * dens_k, with h^2 - r^2 = (float(h^2) - r^2) - h2_err
+  // (h2_err = float(h^2) - h^2).  The constraint is C = rho/rho0 - 1 =
+  // dens_a * S + dens_c0, both constants as float pairs (hi + lo).
+  float h2_err;
+  float dens_k;              // 2^26 / float(h^2)^3: a term is at most 2^26
+  float dens_a_hi, dens_a_lo;    // m W_norm / (rho0 dens_k)
+  float dens_c0_hi, dens_c0_lo;  // m W(0) / rho0 - 1
   float lam_k2;              // (m * DelW_norm / rho0)^2
   float lambda_eps;
   float dt, g_dt;            // g_dt = -g * dt
@@ -68,6 +76,11 @@ struct SdfDev {
   float len[3];       // float(wmax - wmin)
   float inv_dim[3];   // 1/dim (texel size in normalised coordinates)
   float m2w;          // world length / model length (obstacle_cuda.cu:101-105)
+};
+
+// The obstacle's world box in double, for callers holding FP64 positions.
+struct SdfBox64 {
+  double lo[3], hi[3];
 };
 
 // Batched scenes: each scene has its own box (local coordinates from the
@@ -124,7 +137,7 @@ enum KernelId {
   K_PREDICT_HASH = 0, K_HASH, K_GRAVITY, K_PREDICT, K_RADIX_UP, K_RADIX_DOWN, K_SCAN,
   K_REORDER, K_BUILD_LISTS, K_DENSITY_LAMBDA, K_DENSITY, K_LAMBDA, K_DP, K_DP_UPDATE,
   K_POS_STARS, K_VELOCITY, K_COLOR, K_SURFACE, K_XSPH, K_VORTICITY, K_CONFINE, K_COMMIT,
-  K_AUX, K_WARM_LAMBDA, K_COLLIDE, K_COUNT
+  K_AUX, K_WARM_LAMBDA, K_COLLIDE, K_LAMBDA_COPY, K_COUNT
 };
 
 struct ScanScratch {
@@ -176,6 +189,9 @@ struct sph_ctx {
   float4 *pw = nullptr;   // (x*, |omega|) for the confinement gather
   float4 *dpbuf = nullptr;  // staged delta p
   float *rho = nullptr;
+  // stage path: C = rho/rho0 - 1 of the last density stage (read by a lambda
+  // stage that runs its own gather) and the lambda that density pass computed
+  float *cstage = nullptr, *lam_stage = nullptr;
   uint32_t *list = nullptr;     // sorted neighbour indices, quads of slots per 64 particles
   uint4 *list_head = nullptr;   // per sorted particle: layer bases and counts
   // state transfers: SoA x, y, z, vx, vy, vz of up to cap particles, on the
@@ -287,6 +303,8 @@ int launch_commit(sph_ctx *c);
 int launch_substep_init(sph_ctx *c);
 int launch_collide_soa(float *x, float *y, float *z, uint32_t n, const SdfDev &o,
                        hipStream_t s);
+int launch_collide_soa_f64(double *x, double *y, double *z, uint32_t n, const SdfDev &o,
+                           const SdfBox64 &b, hipStream_t s);
 int launch_kinetic_energy(sph_ctx *c, double *out_dev);
 int launch_warm_lambda(sph_ctx *c, bool log);
 int launch_scatter_lambda(sph_ctx *c);

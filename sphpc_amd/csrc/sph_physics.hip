@@ -466,17 +466,35 @@ __global__ __launch_bounds__(kBlock) void k_build_lists(
     }
     // the format is per wavefront (the lanes' quads interleave)
     wide = __ballot(direct) != 0ull;
+    // Distances from the particle to the faces of its own cell: a particle of
+    // the cell below (above) along an axis lies at least lo (hi) away along
+    // it, since its coordinate is below (at or above) that face.  (A particle
+    // hashed onto the grid's edge from outside lies farther still, and a NaN
+    // distance skips nothing.)  The products are exact, each distance is
+    // rounded once; a cell is skipped only when its squared distance reaches
+    // the upper edge of the guard band, so no particle the exact test would
+    // accept is ever skipped and the lists stay those of the 27-cell scan.
+    const float lx = fmaf(-(float)cx, P.h, p.x), hx = fmaf((float)(cx + 1), P.h, -p.x);
+    const float ly = fmaf(-(float)cy, P.h, p.y), hy = fmaf((float)(cy + 1), P.h, -p.y);
+    const float lz = fmaf(-(float)cz, P.h, p.z), hz = fmaf((float)(cz + 1), P.h, -p.z);
+    const float lx2 = lx * lx, hx2 = hx * hx;
     int cl[3] = {0, 0, 0};
     for (int dz = -1; dz <= 1; ++dz) {
       const int z = (int)cz + dz;
       if (z >= 0 && z < (int)G.nz) {
         const uint32_t lb = dz < 0 ? base[0] : (dz == 0 ? base[1] : base[2]);
+        const float dz2 = dz < 0 ? lz * lz : (dz > 0 ? hz * hz : 0.0f);
         for (int dy = -1; dy <= 1; ++dy) {
           const int y = (int)cy + dy;
           if (y < 0 || y >= (int)G.ny) continue;
+          const float d2 = dz2 + (dy < 0 ? ly * ly : (dy > 0 ? hy * hy : 0.0f));
+          if (d2 >= P.h2_hi) continue;  // no cell of this row is within h
           const uint32_t row = G.base + (uint32_t)z * G.nxy + (uint32_t)y * G.nx;
-          const uint32_t j0 = cstart[row + x0];
-          const uint32_t j1 = cstart[row + x1 + 1];
+          // the row's first (last) cell only when it can hold a neighbour
+          const uint32_t xa = (x0 < cx && d2 + lx2 >= P.h2_hi) ? cx : x0;
+          const uint32_t xb = (x1 > cx && d2 + hx2 >= P.h2_hi) ? cx : x1;
+          const uint32_t j0 = cstart[row + xa];
+          const uint32_t j1 = cstart[row + xb + 1];
           // candidates 32 at a time: test all (8 loads in flight per group),
           // then append the hits of the chunk in j order
           for (uint32_t cb = j0; cb < j1; cb += 32) {
@@ -576,15 +594,41 @@ __device__ __forceinline__ bool converged(const uint32_t *stats) {
   return stats[STAT_CONVERGED] != 0u;
 }
 
+// One neighbour's density term: the integer dens_k (h^2 - r^2)^3, truncated,
+// with h^2 - r^2 taken as (float(h^2) - r^2) - h2_err, i.e. without the
+// rounding of float(h^2).  Sums of at most 60 terms of at most 2^26 fit 32
+// bits; the sum is exact and independent of the order of the terms.
+__device__ __forceinline__ uint32_t density_term(float r2, const DevParams &P) {
+  // W(r) = norm (h^2 - r^2)^3, 0 for r >= h
+  const float u = fmaxf((P.h2 - r2) - P.h2_err, 0.0f);
+  return (uint32_t)((u * u) * (u * P.dens_k));
+}
+
+// C = rho/rho0 - 1 from the fixed-point density sum S of cnt terms.  Each
+// truncation drops half a unit on average, restored by cnt/2.  S is split
+// into 20 high and 12 low bits, each exact in FP32, and C = a S + c0 is
+// evaluated with a and c0 as float pairs: near rest C ~ 1e-3 is the
+// difference of two terms ~0.8, and this keeps its error at the FP32
+// rounding of the terms instead of that of rho ~ 1e3.
+__device__ __forceinline__ float density_constraint(uint32_t S, int cnt, const DevParams &P) {
+  S += (uint32_t)cnt >> 1;
+  const float shi = (float)(S & 0xFFFFF000u), slo = (float)(S & 0xFFFu);
+  return fmaf(P.dens_a_hi, shi, P.dens_c0_hi) +
+         (P.dens_c0_lo + fmaf(P.dens_a_lo, shi, P.dens_a_hi * slo));
+}
+
 // ComputeDensities (particles.c:461-514) and ComputeLambda (particles.c:588-666)
-// in one gather: rho_i needs only r^2, lambda_i needs rho_i (own) and the
+// in one gather: rho_i needs only r^2, lambda_i needs C_i (own) and the
 // spiky gradients.  Writes rho and pos.w = lambda.
-// MODE 0: both, 1: density only (rho), 2: lambda only (reads rho).
+// MODE 0: both, 1: density only, 2: lambda only (C from c_io).  With c_io,
+// MODE 0/1 also store C there; with lam_out, lambda goes there and pos.w
+// keeps rho (density_to_w).
 template <int MODE, int NB>
 __global__ __launch_bounds__(kBlock) void k_density_lambda(
     uint32_t n, uint32_t nblocks, float4 *pos, const uint32_t *__restrict__ list,
-    const uint4 *__restrict__ head, DevParams P, float *rho_io, uint32_t *stats, int err_slot,
-    int check_converged, int density_to_w, float *__restrict__ lam_out) {
+    const uint4 *__restrict__ head, DevParams P, float *__restrict__ rho_out, float *c_io,
+    uint32_t *stats, int err_slot, int check_converged, int density_to_w,
+    float *__restrict__ lam_out) {
   if (check_converged && converged(stats)) return;
   const uint32_t s = particle_index(nblocks);
   const bool active = s < n;
@@ -592,15 +636,13 @@ __global__ __launch_bounds__(kBlock) void k_density_lambda(
   if (active) {
     const float4 p = pos[s];
     const NbList nl = nb_list(list, head, s);
-    float su3 = 0.0f, sgx = 0.0f, sgy = 0.0f, sgz = 0.0f, sc = 0.0f;
+    uint32_t S = 0u;
+    float sgx = 0.0f, sgy = 0.0f, sgz = 0.0f, sc = 0.0f;
     for_neighbors<float4, NB, true, true>(nl, s, [&](uint32_t j) { return pos[j]; },
                                           [&](const float4 &q) {
       const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
       const float r2 = sq3(dx, dy, dz);
-      if (MODE != 2) {
-        const float u = fmaxf(P.h2 - r2, 0.0f);  // W(r) = norm (h^2 - r^2)^3, 0 for r >= h
-        su3 = fmaf(u * u, u, su3);
-      }
+      if (MODE != 2) S += density_term(r2, P);
       if (MODE != 1) {
         const float r2c = fmaxf(r2, P.rmin2);
         const float ir = fast_rsq(r2c);
@@ -613,17 +655,21 @@ __global__ __launch_bounds__(kBlock) void k_density_lambda(
         sc = fmaf(g * g, r2, sc);
       }
     });
-    float rho;
+    float C;
     if (MODE != 2) {
-      rho = fmaf(P.mWn, su3, P.mW0);
-      rho_io[s] = rho;
+      C = density_constraint(S, nl.c, P);
+      const float rho = fmaf(C, P.rest_density, P.rest_density);
+      rho_out[s] = rho;
+      if (c_io) c_io[s] = C;
+      // the reported error is that of the stored density, as every later
+      // reader of rho (per-scene errors, the warm fallback) computes it
       err = fmaxf((rho - P.rest_density) * P.inv_rest_density, 0.0f);
       if (density_to_w) pos[s].w = rho;
     } else {
-      rho = rho_io[s];
+      C = c_io[s];
     }
     if (MODE != 1) {
-      const float Ci = fmaxf((rho - P.rest_density) * P.inv_rest_density, 0.0f);
+      const float Ci = fmaxf(C, 0.0f);
       const float sum_c = P.lam_k2 * (sc + sq3(sgx, sgy, sgz));
       const float lam = -Ci / (sum_c + P.lambda_eps);
       if (lam_out)
@@ -660,6 +706,26 @@ __device__ __forceinline__ float sdf_sample(const SdfDev &o, float u, float v, f
   return c0 + aw * (c1 - c0);
 }
 
+// The push of CalculateParticleCollisions_kernel (obstacle_cuda.cu:81-113)
+// at normalised grid coordinates (u, v, w): false outside the obstacle,
+// otherwise the world-space push along the SDF gradient.
+__device__ __forceinline__ bool sdf_push(const SdfDev &o, float u, float v, float w,
+                                         float &px, float &py, float &pz) {
+  const float d = sdf_sample(o, u, v, w);
+  if (!(d < 0.0f)) return false;
+  const float gx = sdf_sample(o, u + o.inv_dim[0], v, w) - sdf_sample(o, u - o.inv_dim[0], v, w);
+  const float gy = sdf_sample(o, u, v + o.inv_dim[1], w) - sdf_sample(o, u, v - o.inv_dim[1], w);
+  const float gz = sdf_sample(o, u, v, w + o.inv_dim[2]) - sdf_sample(o, u, v, w - o.inv_dim[2]);
+  const float dd = gx * gx + gy * gy + gz * gz;
+  if (!(dd > 0.0f)) return false;  // the reference would normalise a zero vector to NaN
+  const float inv = 1.0f / sqrtf(dd);
+  const float dist = fabsf(d) * o.m2w;
+  px = dist * (gx * inv);
+  py = dist * (gy * inv);
+  pz = dist * (gz * inv);
+  return true;
+}
+
 // CalculateParticleCollisions_kernel (obstacle_cuda.cu:56-116).
 __device__ __forceinline__ void collide(float &x, float &y, float &z, const SdfDev &o) {
   if (!o.data) return;
@@ -669,18 +735,11 @@ __device__ __forceinline__ void collide(float &x, float &y, float &z, const SdfD
   const float u = (x - o.wmin[0]) / o.len[0];
   const float v = (y - o.wmin[1]) / o.len[1];
   const float w = (z - o.wmin[2]) / o.len[2];
-  const float d = sdf_sample(o, u, v, w);
-  if (!(d < 0.0f)) return;
-  const float gx = sdf_sample(o, u + o.inv_dim[0], v, w) - sdf_sample(o, u - o.inv_dim[0], v, w);
-  const float gy = sdf_sample(o, u, v + o.inv_dim[1], w) - sdf_sample(o, u, v - o.inv_dim[1], w);
-  const float gz = sdf_sample(o, u, v, w + o.inv_dim[2]) - sdf_sample(o, u, v, w - o.inv_dim[2]);
-  const float dd = gx * gx + gy * gy + gz * gz;
-  if (!(dd > 0.0f)) return;  // the reference would normalise a zero vector to NaN
-  const float inv = 1.0f / sqrtf(dd);
-  const float dist = fabsf(d) * o.m2w;
-  y += dist * (gy * inv);
-  x += dist * (gx * inv);
-  z += dist * (gz * inv);
+  float px, py, pz;
+  if (!sdf_push(o, u, v, w, px, py, pz)) return;
+  y += py;
+  x += px;
+  z += pz;
 }
 
 // UpdateDPs (particles.c:668-746): returns sum_j (lambda_i + lambda_j) DelW(r)/r d.
@@ -837,16 +896,17 @@ __global__ __launch_bounds__(kBlock) void k_warm_lambda(
     const float4 po = xold[s];
     const float ux = p.x - po.x, uy = p.y - po.y, uz = p.z - po.z;  // dt * v_i
     const NbList nl = nb_list(list, head, s);
-    float su3 = 0.0f, sgx = 0.0f, sgy = 0.0f, sgz = 0.0f, sc = 0.0f;
+    uint32_t S = 0u;
+    float sgx = 0.0f, sgy = 0.0f, sgz = 0.0f, sc = 0.0f;
     float mx = 0.0f, my = 0.0f, mz = 0.0f, dv = 0.0f;
     auto load = [&](uint32_t j) { return Pair4{pos[j], xold[j]}; };
     for_neighbors<Pair4, NB, false>(nl, s, load, [&](const Pair4 &qq) {
       const float4 &q = qq.a, &qo = qq.b;
       const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
       const float r2 = sq3(dx, dy, dz);
+      S += density_term(r2, P);  // as k_density_lambda
       const float u = fmaxf(P.h2 - r2, 0.0f);
       const float w3 = u * u * u;
-      su3 = fmaf(u * u, u, su3);  // as k_density_lambda accumulates it
       mx = fmaf(-w3, dx, mx);
       my = fmaf(-w3, dy, my);
       mz = fmaf(-w3, dz, mz);
@@ -862,13 +922,13 @@ __global__ __launch_bounds__(kBlock) void k_warm_lambda(
       const float ex = (q.x - qo.x) - ux, ey = (q.y - qo.y) - uy, ez = (q.z - qo.z) - uz;
       dv = fmaf(g, fmaf(ez, dz, fmaf(ey, dy, ex * dx)), dv);
     });
-    rho = fmaf(P.mWn, su3, P.mW0);
+    const float cr = density_constraint(S, nl.c, P);
+    rho = fmaf(cr, P.rest_density, P.rest_density);
     rho_io[s] = rho;
-    const float cr = (rho - P.rest_density) * P.inv_rest_density;
-    err = fmaxf(cr, 0.0f);
+    err = fmaxf((rho - P.rest_density) * P.inv_rest_density, 0.0f);  // as k_density_lambda
     const float gc2 = P.lam_k2 * (sc + sq3(sgx, sgy, sgz));
     denom = gc2 + P.lambda_eps;
-    lam0 = -err / denom;
+    lam0 = -fmaxf(cr, 0.0f) / denom;  // as k_density_lambda
     f[0] = (float)nl.c * (1.0f / 32.0f);
     f[1] = cr;
     f[2] = -lam_prev[id_s[s]] * denom;
@@ -1258,6 +1318,30 @@ __global__ __launch_bounds__(kBlock) void k_collide_soa(uint32_t n, float *__res
   z[i] = c;
 }
 
+// CalculateParticleCollisions_kernel on the reference's own FP64 arrays
+// (obstacle_cuda.cu:56-116): the bounds test and the normalised coordinates
+// in double against the double world box, the SDF arithmetic in FP32, and
+// the FP32 push added to the double coordinate.
+__global__ __launch_bounds__(kBlock) void k_collide_soa_f64(uint32_t n, double *__restrict__ x,
+                                                            double *__restrict__ y,
+                                                            double *__restrict__ z, SdfDev o,
+                                                            SdfBox64 b) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double xw = x[i], yw = y[i], zw = z[i];
+  if (!(xw > b.lo[0] && xw < b.hi[0] && yw > b.lo[1] && yw < b.hi[1] && zw > b.lo[2] &&
+        zw < b.hi[2]))
+    return;
+  const float u = (float)((xw - b.lo[0]) / (double)o.len[0]);
+  const float v = (float)((yw - b.lo[1]) / (double)o.len[1]);
+  const float w = (float)((zw - b.lo[2]) / (double)o.len[2]);
+  float px, py, pz;
+  if (!sdf_push(o, u, v, w, px, py, pz)) return;
+  y[i] = yw + (double)py;
+  x[i] = xw + (double)px;
+  z[i] = zw + (double)pz;
+}
+
 // CalculateParticleCollisions over the sorted positions (after a k_dp_update
 // without the fused push).  Only particles inside the obstacle's box are
 // written.
@@ -1482,40 +1566,40 @@ int launch_density(sph_ctx *c, int mode, int err_slot) {
       constexpr int NBv = decltype(D_)::value;
       LAUNCH_TIMED(c, K_DENSITY_LAMBDA, 0,
                    k_density_lambda<0, NBv><<<nb, bs, 0, s>>>(
-                       n, nb, c->pos, c->list, c->list_head, c->dev, c->rho, c->stats, err_slot,
-                       c->P.density_tolerance > 0, 0, nullptr));
+                       n, nb, c->pos, c->list, c->list_head, c->dev, c->rho, nullptr, c->stats,
+                       err_slot, c->P.density_tolerance > 0, 0, nullptr));
     });
   else if (mode == 1)
     with_depth(tuned_depth(c, K_DENSITY), [&](auto D_) {
       constexpr int NBv = decltype(D_)::value;
       LAUNCH_TIMED(c, K_DENSITY, 0,
-                   k_density_lambda<1, NBv><<<nb, bs, 0, s>>>(n, nb, c->pos, c->list, c->list_head,
-                                                             c->dev, c->rho, c->stats, err_slot, 0,
-                                                             0, nullptr));
+                   k_density_lambda<1, NBv><<<nb, bs, 0, s>>>(
+                       n, nb, c->pos, c->list, c->list_head, c->dev, c->rho, c->cstage, c->stats,
+                       err_slot, 0, 0, nullptr));
     });
   else if (mode == 4)  // density and lambda in one gather: rho into w, lambda aside
     with_depth(tuned_depth(c, K_DENSITY_LAMBDA), [&](auto D_) {
       constexpr int NBv = decltype(D_)::value;
       LAUNCH_TIMED(c, K_DENSITY_LAMBDA, 0,
                    k_density_lambda<0, NBv><<<nb, bs, 0, s>>>(
-                       n, nb, c->pos, c->list, c->list_head, c->dev, c->rho, c->stats,
-                       err_slot, 0, 1, reinterpret_cast<float *>(c->dpbuf)));
+                       n, nb, c->pos, c->list, c->list_head, c->dev, c->rho, c->cstage, c->stats,
+                       err_slot, 0, 1, c->lam_stage));
     });
   else if (mode == 3)  // final density of a capped tolerance loop: rho also into w
     with_depth(tuned_depth(c, K_DENSITY), [&](auto D_) {
       constexpr int NBv = decltype(D_)::value;
       LAUNCH_TIMED(c, K_DENSITY, 0,
-                   k_density_lambda<1, NBv><<<nb, bs, 0, s>>>(n, nb, c->pos, c->list, c->list_head,
-                                                             c->dev, c->rho, c->stats, err_slot, 1,
-                                                             1, nullptr));
+                   k_density_lambda<1, NBv><<<nb, bs, 0, s>>>(
+                       n, nb, c->pos, c->list, c->list_head, c->dev, c->rho, c->cstage, c->stats,
+                       err_slot, 1, 1, nullptr));
     });
   else
     with_depth(tuned_depth(c, K_LAMBDA), [&](auto D_) {
       constexpr int NBv = decltype(D_)::value;
       LAUNCH_TIMED(c, K_LAMBDA, 0,
-                   k_density_lambda<2, NBv><<<nb, bs, 0, s>>>(n, nb, c->pos, c->list, c->list_head,
-                                                             c->dev, c->rho, c->stats, -1, 0, 0,
-                                                             nullptr));
+                   k_density_lambda<2, NBv><<<nb, bs, 0, s>>>(
+                       n, nb, c->pos, c->list, c->list_head, c->dev, c->rho, c->cstage, c->stats,
+                       -1, 0, 0, nullptr));
     });
   SPH_HIP(hipGetLastError());
   return SPH_OK;
@@ -1531,9 +1615,8 @@ __global__ __launch_bounds__(kBlock) void k_lambda_to_w(uint32_t n, const float 
 int launch_lambda_from_density(sph_ctx *c) {
   const uint32_t n = c->n;
   if (!n) return SPH_OK;
-  LAUNCH_TIMED(c, K_LAMBDA, 0,
-               k_lambda_to_w<<<blocks_for(n), kBlock, 0, c->stream>>>(
-                   n, reinterpret_cast<const float *>(c->dpbuf), c->pos));
+  LAUNCH_TIMED(c, K_LAMBDA_COPY, 0,
+               k_lambda_to_w<<<blocks_for(n), kBlock, 0, c->stream>>>(n, c->lam_stage, c->pos));
   SPH_HIP(hipGetLastError());
   return SPH_OK;
 }
@@ -1838,6 +1921,14 @@ int launch_collide_soa(float *x, float *y, float *z, uint32_t n, const SdfDev &o
                        hipStream_t s) {
   if (!n) return SPH_OK;
   k_collide_soa<<<blocks_for(n), kBlock, 0, s>>>(n, x, y, z, o);
+  SPH_HIP(hipGetLastError());
+  return SPH_OK;
+}
+
+int launch_collide_soa_f64(double *x, double *y, double *z, uint32_t n, const SdfDev &o,
+                           const SdfBox64 &b, hipStream_t s) {
+  if (!n) return SPH_OK;
+  k_collide_soa_f64<<<blocks_for(n), kBlock, 0, s>>>(n, x, y, z, o, b);
   SPH_HIP(hipGetLastError());
   return SPH_OK;
 }

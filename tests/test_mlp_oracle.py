@@ -92,3 +92,78 @@ def test_bf16_forward_close_to_fp64_forward():
     yb = M.forward_bf16(theta, xn)
     assert np.max(np.abs(yb - y64)) < 0.05 * np.max(np.abs(y64))
     assert np.max(np.abs(yb - y64)) > 0  # the rounding is really applied
+
+
+# ---------------------------------------------------------------------------
+# Closed forms of the warm-start features the reference has no code for
+# ---------------------------------------------------------------------------
+def _lattice64(n, s, origin=0.1):
+    """Perfect cell-centred lattice in FP64, x fastest (n^3 particles)."""
+    iz, iy, ix = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    return origin + (np.stack([ix.ravel(), iy.ravel(), iz.ravel()], 1) + 0.5) * s
+
+
+def _offsets():
+    import itertools
+    return [k for k in itertools.product(range(-2, 3), repeat=3)
+            if 0 < k[0] ** 2 + k[1] ** 2 + k[2] ** 2 < 4]
+
+
+def _features(xs, xo, s):
+    import orc
+    p = orc.make_params(s, (1.0, 1.0, 1.0))
+    X = [np.ascontiguousarray(xs[:, a]) for a in range(3)]
+    nbr, cnt, dropped = orc.neighbors(p, *X)
+    assert dropped == 0
+    n = len(xs)
+    f = M.warm_features(xs, xo, nbr.reshape(n, orc.MAXN), cnt, np.zeros(n), p.h, p.rest_mass,
+                        p.rest_density, p.lambda_eps)
+    return f, p
+
+
+def test_velocity_divergence_feature_uniform_expansion():
+    # Feature 4 is (m/rho0) DelW_norm sum_j (h - r)^2/r ((dt v_j - dt v_i) . d),
+    # d = x*_i - x*_j.  Under a uniform expansion v = a (x - c) the predicted
+    # lattice is x* = c + (1 + a dt)(x - c), so dt (v_j - v_i) = -a dt/(1 + a dt) d
+    # and the feature is -a dt/(1 + a dt) (m/rho0) sum_k DelW(r_k) r_k with
+    # r_k = (1 + a dt) s |k| over the lattice offsets inside h = 2s.
+    s, a, dt = 0.02, 3.0, 1.0 / 300.0
+    xo = _lattice64(12, s)
+    c = xo.mean(0)
+    xs = c + (1.0 + a * dt) * (xo - c)
+    f, p = _features(xs, xo, s)
+    h, m, rho0 = p.h, p.rest_mass, p.rest_density
+    lat = 0.0
+    for k in _offsets():
+        r = (1.0 + a * dt) * s * np.sqrt(k[0] ** 2 + k[1] ** 2 + k[2] ** 2)
+        if r < h:
+            lat += -45.0 / (np.pi * h ** 6) * (h - r) ** 2 * r
+    expected = -a * dt / (1.0 + a * dt) * m / rho0 * lat
+    idx = np.arange(len(xs)).reshape(12, 12, 12)[4:8, 4:8, 4:8].ravel()
+    assert np.max(np.abs(f[idx, 4] - expected)) < 1e-12 * abs(expected)
+    # the SPH estimate of dt div v = 3 a dt, to the lattice's discretisation error
+    assert abs(expected / (3.0 * a * dt) - 1.0) < 0.15
+    # a rigid translation has no divergence
+    f2, _ = _features(xo + np.array([0.01, -0.02, 0.005]), xo, s)
+    assert np.max(np.abs(f2[:, 4])) < 1e-12
+
+
+def test_density_moment_feature_half_space():
+    # Features 9-11 are -(m W_norm/(rho h)) sum_j (h^2 - r^2)^3 (x*_i - x*_j) with
+    # rho the particle's own density.  On a perfect lattice (h = 2s) the terms
+    # are s^6 (4 - |k|^2)^3 at offsets s k, so the moment is
+    # (s/h) sum (4-|k|^2)^3 k / (64 + sum (4-|k|^2)^3): zero in the interior and
+    # (0, 0, -63/267)/2 for a particle of the top layer (the k_z = -1 layer
+    # sums to 27 + 4*8 + 4*1 = 63, the k_z <= 0 half to 140 + 63, plus 64 for
+    # the particle itself).
+    s = float(np.float32(0.02))  # h = 2 float(s) exactly, as the parameters derive it
+    xs = _lattice64(12, s)
+    f, _ = _features(xs, xs, s)
+    grid = np.arange(len(xs)).reshape(12, 12, 12)
+    inner = grid[4:8, 4:8, 4:8].ravel()
+    top = grid[-1, 4:8, 4:8].ravel()
+    assert np.max(np.abs(f[inner, 9:12])) < 1e-12
+    expected = np.array([0.0, 0.0, -63.0 / 267.0 / 2.0])
+    assert np.max(np.abs(f[top, 9:12] - expected)) < 1e-12
+    # feature 1: the same particle's C = rho/rho0 - 1 with rho/rho_interior = 267/330
+    assert np.allclose((f[top, 1] + 1.0) / (f[inner[0], 1] + 1.0), 267.0 / 330.0, rtol=1e-12)

@@ -285,11 +285,21 @@ def test_galilean_invariance_of_velocity_passes():
         assert np.allclose(w1[i], w2[i], atol=1e-9 * scale)
 
 
-def test_rigid_rotation_vorticity_direction():
-    # v = Omega x r: the reference sum (v_j - v_i) x gradW is parallel to Omega
-    # for interior particles (vort_model 1 gives ~2*Omega up to the kernel sum).
+def _del_w(r, h):
+    """Spiky gradient magnitude DelW(r) = -45/(pi h^6) (h - r)^2 (particles.c:40-46)."""
+    return -45.0 / (math.pi * h ** 6) * (h - r) ** 2 if r < h else 0.0
+
+
+def test_rigid_rotation_vorticity_closed_form():
+    # v = Omega x r on a perfect lattice.  With d = x_i - x_j, each term is
+    # w_j (v_j - v_i) x DelW(r) d/r = w_j DelW(r)/r (r^2 Omega - d (Omega.d)), and
+    # cubic symmetry turns sum f(r) d d^T into (1/3) sum f(r) r^2 I, so
+    # omega_i = (2/3) sum_k w DelW(s|k|) s|k| Omega over lattice offsets k with
+    # |k|^2 < 4 (h = 2s), w = 1 (vort_model 0, particles.c:207-283) or m/rho
+    # with rho = m W_norm s^6 * 330 (model 1).
     p, xs, ys, zs, nbr, cnt = _jittered(9, n=(12, 12, 12), jitter=0.0)
     n = len(xs)
+    s, h, m = p.h / 2.0, p.h, p.rest_mass
     rho = np.zeros(n)
     L.orc_compute_densities(n, xs, ys, zs, nbr, cnt, C.byref(p), rho)
     om = np.array([0.3, -0.2, 1.1])
@@ -297,16 +307,79 @@ def test_rigid_rotation_vorticity_direction():
     v = np.cross(om, r)
     vx, vy, vz = [np.ascontiguousarray(v[:, i]) for i in range(3)]
     idx = np.arange(n).reshape(12, 12, 12)[4:8, 4:8, 4:8].ravel()
+    lat = sum(_del_w(s * math.sqrt(k[0] ** 2 + k[1] ** 2 + k[2] ** 2), h) *
+              s * math.sqrt(k[0] ** 2 + k[1] ** 2 + k[2] ** 2) for k in _lattice_offsets(4))
+    rho_lat = m * 315.0 / (64.0 * math.pi * h ** 9) * s ** 6 * 330.0
+    for model, w in ((0, 1.0), (1, m / rho_lat)):
+        p.vort_model = model
+        out = [np.zeros(n) for _ in range(3)]
+        L.orc_compute_vorticity(n, xs, ys, zs, rho, vx, vy, vz, nbr, cnt, C.byref(p), *out)
+        W = np.stack(out, 1)[idx]
+        expected = (2.0 / 3.0) * w * lat * om
+        # float32 lattice positions: ~1e-7 relative asymmetry per pair
+        assert np.max(np.abs(W - expected)) < 1e-5 * np.linalg.norm(expected), model
+    # model 1 approximates the curl of the rotation, 2 Omega, to the lattice's
+    # discretisation error (0.947 of it here; antiparallel, because the
+    # reference's gradient is taken along x_i - x_j)
+    assert abs((2.0 / 3.0) * (m / rho_lat) * lat / -2.0 - 1.0) < 0.1
+
+
+def test_vorticity_two_particles_closed_form():
+    # Two particles along x at r = 0.6 h, v_j - v_i = (0, 0, w): omega_i =
+    # (0, -w DelW(r) w_j, 0) and omega_j = (0, -w DelW(r) w_i, 0), with
+    # w_q = 1 (model 0) or m / rho_q (model 1).  Different densities at i and
+    # j pin that the weight is the neighbour's.
+    s = 0.02
+    p = orc.make_params(s, (0.5, 0.5, 0.5), vort_eps=0.01)
+    h, m = p.h, p.rest_mass
+    r = 0.6 * h
+    xs = np.array([0.2, 0.2 + r]); ys = np.array([0.2, 0.2]); zs = np.array([0.2, 0.2])
+    vx = np.array([0.5, 0.5]); vy = np.array([-0.1, -0.1]); vz = np.array([0.0, 0.7])
+    rho = np.array([1200.0, 800.0])
+    nbr = np.zeros(2 * orc.MAXN, np.uint32)
+    nbr[0], nbr[orc.MAXN] = 1, 0
+    cnt = np.array([1, 1], np.int32)
+    dw = _del_w(r, h)
     for model in (0, 1):
         p.vort_model = model
-        w = [np.zeros(n) for _ in range(3)]
-        L.orc_compute_vorticity(n, xs, ys, zs, rho, vx, vy, vz, nbr, cnt, C.byref(p), *w)
-        W = np.stack(w, 1)[idx]
-        cos = W @ om / (np.linalg.norm(W, axis=1) * np.linalg.norm(om))
-        assert np.min(np.abs(cos)) > 1 - 1e-9
-        if model == 1:
-            ratio = np.linalg.norm(W, axis=1) / np.linalg.norm(2 * om)
-            assert np.all(np.abs(ratio - 1) < 0.1)
+        w = [np.zeros(2) for _ in range(3)]
+        L.orc_compute_vorticity(2, xs, ys, zs, rho, vx, vy, vz, nbr, cnt, C.byref(p), *w)
+        wi = 1.0 if model == 0 else m / rho[1]
+        wj = 1.0 if model == 0 else m / rho[0]
+        assert abs(w[1][0] - (-0.7 * dw * wi)) < 1e-12 * abs(0.7 * dw * wi)
+        assert abs(w[1][1] - (-0.7 * dw * wj)) < 1e-12 * abs(0.7 * dw * wj)
+        for a in (0, 2):
+            assert w[a][0] == 0.0 and w[a][1] == 0.0
+
+
+def test_confinement_three_particles_closed_form():
+    # i at the origin of an L: j at +r x, k at +r y (r = 0.6 h), with
+    # |omega_j| = 3, |omega_k| = 1, omega_i = (0, 0, 2).  eta_i =
+    # sum_q |omega_q| DelW(r) (x_i - x_q)/r = |DelW(r)| (3, 1, 0) (DelW < 0),
+    # so N_i = (3, 1, 0)/sqrt(10) and N x omega_i = (2, -6, 0)/sqrt(10):
+    # dv_i = dt eps (2, -6, 0)/sqrt(10) / m for model 0 (particles.c:375-379)
+    # and without the 1/m for model 1.  Using |omega_i| for eta would give
+    # N = (1, 1, 0)/sqrt(2), a missing or extra 1/m a factor m.
+    s = 0.02
+    p = orc.make_params(s, (0.5, 0.5, 0.5), vort_eps=0.01)
+    h, m, dt, eps = p.h, p.rest_mass, p.dt, p.vort_eps
+    r = 0.6 * h
+    xs = np.array([0.2, 0.2 + r, 0.2]); ys = np.array([0.2, 0.2, 0.2 + r])
+    zs = np.array([0.2, 0.2, 0.2])
+    wx = np.array([0.0, 0.0, 1.0]); wy = np.array([0.0, 0.0, 0.0]); wz = np.array([2.0, -3.0, 0.0])
+    nbr = np.zeros(3 * orc.MAXN, np.uint32)
+    for i, js in enumerate(([1, 2], [0, 2], [0, 1])):
+        nbr[i * orc.MAXN:i * orc.MAXN + 2] = js
+    cnt = np.array([2, 2, 2], np.int32)
+    v0 = (np.array([0.1, 0.0, 0.0]), np.array([0.2, 0.0, 0.0]), np.array([0.3, 0.0, 0.0]))
+    for model, mass_p in ((0, m), (1, 1.0)):
+        p.vort_model = model
+        v = [a.copy() for a in v0]
+        L.orc_apply_vorticity_confinement(3, xs, ys, zs, wx, wy, wz, nbr, cnt, C.byref(p), *v)
+        k = dt * eps / math.sqrt(10.0) / mass_p
+        expected = (2.0 * k, -6.0 * k, 0.0)
+        for a in range(3):
+            assert abs((v[a][0] - v0[a][0]) - expected[a]) < 1e-12 * 6.0 * k, (model, a)
 
 
 def test_max_density_error_compression_only():
