@@ -212,6 +212,17 @@ int sph_sort_by_key(uint32_t *keys, uint32_t *values, int n, void *stream);
 /* Same, sorting only the low `key_bits` bits (keys must be < 2^key_bits). */
 int sph_sort_by_key_bits(uint32_t *keys, uint32_t *values, int n, int key_bits,
                          void *stream);
+/* No counterpart in the reference: sph_sort_by_key_bits for callers that
+ * cannot allocate device memory (code under stream capture, serving loops,
+ * harnesses that forbid hipMalloc).  Its scratch is a file-scope device array
+ * sized for SPH_SORT_NOALLOC_MAX_PAIRS pairs, which every GPU the library runs
+ * on holds until the process ends; larger n is rejected (SPH_ERR_ARG).  The
+ * work is only enqueued on `stream` (no synchronize).  All calls on one GPU
+ * share that array, so calls on one GPU that overlap (different streams
+ * without ordering, or different threads) race. */
+#define SPH_SORT_NOALLOC_MAX_PAIRS (1 << 21)
+int sph_sort_by_key_bits_noalloc(uint32_t *keys, uint32_t *values, int n, int key_bits,
+                                 void *stream);
 /* FindLowerBounds/FindUpperBounds, thrust_c.h:52-56,65-69: for every cell
  * c in [0, cells) the lower/upper bound of c in n sorted keys. */
 int sph_find_lower_bounds(const uint32_t *keys, int n, int cells, uint32_t *out,

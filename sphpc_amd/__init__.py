@@ -114,6 +114,7 @@ def lib():
     L.sph_download_cell_bounds.argtypes = [vp, up, up]
     L.sph_sort_by_key.argtypes = [vp, vp, ip, vp]
     L.sph_sort_by_key_bits.argtypes = [vp, vp, ip, ip, vp]
+    L.sph_sort_by_key_bits_noalloc.argtypes = [vp, vp, ip, ip, vp]
     L.sph_find_lower_bounds.argtypes = [vp, ip, ip, vp, vp]
     L.sph_find_upper_bounds.argtypes = [vp, ip, ip, vp, vp]
     L.sph_sdf_create.argtypes = [C.POINTER(ObstacleDesc), C.POINTER(vp)]

@@ -507,6 +507,85 @@ extern "C" int sph_sort_by_key(uint32_t *keys, uint32_t *values, int n, void *st
   return sort_impl(keys, values, n, 0, stream);
 }
 
+// --------------------------------------------------------------------------
+// Sort without allocation: scratch in a file-scope device array
+// --------------------------------------------------------------------------
+// For callers that cannot allocate device memory (code under stream capture,
+// serving loops, harnesses that forbid hipMalloc).  Layout, in words: the two
+// ping-pong buffers (kStaticPairs each), the digit histogram (256 per 4096-key
+// tile) and the scan's tile sums; each part starts on a 16-byte boundary.
+namespace {
+constexpr size_t kStaticPairs = size_t(1) << 21;
+constexpr size_t kStaticTiles = (kStaticPairs + RS_TILE - 1) / RS_TILE;
+constexpr size_t kStaticHist = 256 * kStaticTiles;
+constexpr size_t kStaticScanTiles = (kStaticHist + SC_TILE - 1) / SC_TILE;
+constexpr size_t kStaticWords = 2 * kStaticPairs + kStaticHist + ((kStaticScanTiles + 3) & ~3ull);
+static_assert(kStaticPairs % 4 == 0 && kStaticHist % 4 == 0, "16-byte aligned parts");
+static_assert(kStaticPairs == SPH_SORT_NOALLOC_MAX_PAIRS, "header limit");
+}  // namespace
+
+// Each GPU holds its own copy until the process ends (about 17 MB).
+static __device__ __align__(256) uint32_t scratch[kStaticWords];
+
+namespace {
+// Device address of `scratch` on each device, looked up once per device with
+// that device current (the host-side name is not a device address).
+std::mutex g_static_m;
+uint32_t *g_static_ptr[kAbiDevices] = {};
+
+int static_scratch(uint32_t **out) {
+  int dev = 0;
+  SPH_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kAbiDevices) {
+    set_error("device index beyond the native entry points' scratch table");
+    return SPH_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lock(g_static_m);
+  if (!g_static_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress (sort scratch)");
+    g_static_ptr[dev] = static_cast<uint32_t *>(p);
+  }
+  *out = g_static_ptr[dev];
+  return SPH_OK;
+}
+}  // namespace
+
+extern "C" int sph_sort_by_key_bits_noalloc(uint32_t *keys, uint32_t *values, int n,
+                                            int key_bits, void *stream) {
+  if (n < 0 || (n > 0 && (!keys || !values)) || key_bits < 1 || key_bits > 32) {
+    set_error("sph_sort_by_key_bits_noalloc: invalid argument");
+    return SPH_ERR_ARG;
+  }
+  if ((size_t)n > kStaticPairs) {
+    set_error("sph_sort_by_key_bits_noalloc: n exceeds SPH_SORT_NOALLOC_MAX_PAIRS (2^21)");
+    return SPH_ERR_ARG;
+  }
+  if (n <= 1) return SPH_OK;
+  uint32_t *base = nullptr;
+  int rc = static_scratch(&base);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  uint32_t *a = base, *b = base + kStaticPairs;
+  uint32_t *hist = base + 2 * kStaticPairs;
+  size_t hist_cap = kStaticHist;  // large enough: radix_sort_pairs allocates nothing
+  ScanScratch sc;
+  sc.tile_sums = hist + kStaticHist;
+  sc.tile_cap = kStaticScanTiles;
+  const uint32_t *kout = nullptr, *vout = nullptr;
+  rc = radix_sort_pairs(keys, values, a, b, keys, values, (uint32_t)n, key_bits, &hist,
+                        &hist_cap, &sc, s, nullptr, &kout, &vout);
+  if (rc) return rc;
+  if (kout != keys) {  // an odd number of passes ends in the scratch buffers
+    SPH_HIP(hipMemcpyAsync(keys, kout, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToDevice,
+                           s));
+    SPH_HIP(hipMemcpyAsync(values, vout, sizeof(uint32_t) * (size_t)n,
+                           hipMemcpyDeviceToDevice, s));
+  }
+  return SPH_OK;
+}
+
 static int bounds_impl(const uint32_t *keys, int n, int cells, uint32_t *out, void *stream,
                        bool upper) {
   if (n < 0 || cells < 0 || (cells > 0 && !out) || (n > 0 && !keys)) {
