@@ -216,55 +216,132 @@ def compact_tuning(t):
             "gathers_in_flight": {k: t["depth"][k] for k in GATHER_KERNELS if k != "build_lists"}}
 
 
-def bench_batch(args):
-    """Batched scenes: `--scenes` independent scenes in one context, one launch
-    per stage for all of them (sphpc_amd/workloads.py)."""
+def _err_summary(errs):
+    return {"mean": float(np.mean(errs)), "median": float(np.median(errs)),
+            "max": float(np.max(errs))}
+
+
+def run_batch(scenes_count, steps, warmup, graph=True, autotune=True, net_path=None):
+    """Batched scenes: `scenes_count` independent scenes in one context, one
+    launch per stage for all of them (sphpc_amd/workloads.py), 4 Jacobi
+    iterations.  Times `steps` graph-replayed substeps plain and, with a
+    network file, the same substeps from the same state warm-started (the
+    first iteration's multiplier from the network, plain fallback per scene).
+    Returns the result dict."""
     import sphpc_amd as sa
     from sphpc_amd import workloads
-    s, scenes, pos = workloads.batched_scenes(args.scenes, seed=4)
+    s, scenes, pos = workloads.batched_scenes(scenes_count, seed=4)
     n = len(pos)
     p = sa.default_params(2.0 * s, time_step=1.0 / 300.0, g=9.81, k=0.5, c=0.01,
                           rest_density=1000.0, rest_mass=1000.0 * s ** 3, lambda_epsilon=600.0,
                           vorticity_epsilon=0.01, vorticity_model=1, solve_iterations=4)
     ctx = sa.BatchContext(p, scenes)
     ctx.upload(pos)
-    ctx.step(args.warmup)
+    ctx.step(warmup)
     ctx.synchronize()
-    tuning = None if args.no_autotune else ctx.autotune(3)
-    ctx.set_graph(not args.no_graph)
+    tuning = ctx.autotune(3) if autotune else None
+    start = ctx.download()
+
+    def timed(warm_net):
+        ctx.upload(*start)
+        ctx.set_warm_start(warm_net)
+        ctx.set_graph(graph)
+        ctx.step(1)  # capture (untimed)
+        ctx.upload(*start)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        worst = np.zeros(len(scenes))
+        fallbacks = 0
+        for _ in range(steps):
+            ctx.step(1)
+            worst = np.maximum(worst, ctx.scene_errors())
+            fallbacks += ctx.stats().warm_fallback
+        ctx.synchronize()
+        el = time.perf_counter() - t0
+        ctx.set_graph(False)
+        ctx.set_warm_start(None)
+        return el, worst, fallbacks
+
+    # rate without per-substep host reads, then per-scene errors substep by substep
+    ctx.upload(*start)
+    ctx.set_graph(graph)
     ctx.step(1)
+    ctx.upload(*start)
     ctx.synchronize()
     t0 = time.perf_counter()
-    ctx.step(args.steps)
+    ctx.step(steps)
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
-    errs = ctx.scene_errors()
+    ctx.set_graph(False)
+    _, worst_plain, _ = timed(None)
+    warm = None
+    if net_path and os.path.exists(net_path):
+        net = sa.Mlp()
+        net.load(net_path)
+        el_w, worst_warm, fb = timed(net)
+        warm = {"particle_updates_per_s_with_error_reads": n * steps / el_w,
+                "scene_max_density_error": _err_summary(worst_warm),
+                "scenes_better_than_plain": int(np.sum(worst_warm < worst_plain)),
+                "substeps_with_a_fallback": int(fb),
+                "network": os.path.relpath(net_path, ROOT)}
+    ctx.upload(*start)
     st = ctx.stats()
-    nbar = st.neighbor_pairs / max(n, 1)
-    prof_steps = max(2, args.steps // 5)
+    prof_steps = max(2, steps // 5)
     _, times = timed_region(ctx, prof_steps)
-    roof, per = roofline(times, n, nbar, prof_steps, f"pbf_substep_batch{args.scenes}")
+    nbar = ctx.stats().neighbor_pairs / max(n, 1)
+    roof, per = roofline(times, n, nbar, prof_steps, f"pbf_substep_batch{scenes_count}")
     counts = [sc["particles"] for sc in scenes]
-    out = {
-        "metric": f"particle-updates/s, {args.scenes} batched scenes",
-        "value": n * args.steps / elapsed, "unit": "particle-updates/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps,
+    ctx.close()
+    return {
+        "metric": f"particle-updates/s, {scenes_count} batched scenes",
+        "value": n * steps / elapsed, "unit": "particle-updates/s", "n_gpus": 1,
+        "steps": steps, "warmup": warmup, "ms_per_step": 1e3 * elapsed / steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"{args.scenes} independent seeded scenes (2k-128k particles, "
+        "config": {"workload": f"{scenes_count} independent seeded scenes (2k-128k particles, "
                                "box aspect 1-4, fill 0.1-0.5, sphere SDF 64^3), s=0.01 m, "
                                "4 Jacobi iterations, surface tension + XSPH + vorticity",
-                   "particles": n, "scenes": args.scenes, "min_scene": int(min(counts)),
-                   "max_scene": int(max(counts)), "graph": not args.no_graph,
-                   "autotune": compact_tuning(tuning),
-                   "mean_neighbors": nbar,
-                   "scene_max_density_error": {"mean": float(np.mean(errs)),
-                                               "median": float(np.median(errs)),
-                                               "max": float(np.max(errs))}},
+                   "particles": n, "scenes": scenes_count, "min_scene": int(min(counts)),
+                   "max_scene": int(max(counts)), "graph": graph,
+                   "autotune": compact_tuning(tuning), "mean_neighbors": nbar,
+                   "neighbor_overflow": st.neighbor_overflow},
+        "scene_max_density_error_plain": _err_summary(worst_plain),
+        "warm_start": warm,
         "roofline": roof, "kernels_ms_per_substep_eager": per,
     }
-    ctx.close()
+
+
+def bench_batch(args):
+    out = run_batch(args.scenes, args.steps, args.warmup, graph=not args.no_graph,
+                    autotune=not args.no_autotune, net_path=args.batch_net)
     print(json.dumps(out), flush=True)
+
+
+def later_window(ctx, start, first, count, graph):
+    """Throughput over substeps [first, first + count) of the run, replayed
+    from the state saved at `first` (the fused substep from `start`, the state
+    at substep `start_at`)."""
+    ctx.upload(*start[1])
+    ctx.set_graph(False)
+    if first > start[0]:
+        ctx.step(first - start[0])
+    saved = ctx.download()
+    ctx.set_graph(graph)
+    ctx.step(1)
+    ctx.upload(*saved)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.step(count)
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    ctx.set_graph(False)
+    st = ctx.stats()
+    n = ctx.count()
+    return {"substeps": [first, first + count], "value": n * count / el,
+            "ms_per_step": 1e3 * el / count,
+            "mean_neighbors": st.neighbor_pairs / max(n, 1),
+            "max_density_error_last": st.max_density_error,
+            "neighbor_overflow_last": st.neighbor_overflow}
 
 
 def main():
@@ -291,6 +368,15 @@ def main():
                     help="default launch configuration instead of the autotuner's")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run two 1 GiB device copies (counter calibration)")
+    ap.add_argument("--batch-net", default=os.path.join(ROOT, "sphpc_amd", "data",
+                                                        "warm_net_s0.008.bin"),
+                    help="network for the warm-started batch run ('' for none)")
+    ap.add_argument("--prev-scale", type=float, default=1.0,
+                    help="scale of the previous-multiplier warm start (0: not run)")
+    ap.add_argument("--eq-steps", type=int, default=5,
+                    help="substeps per iteration count in the equal-accuracy comparison")
+    ap.add_argument("--batch-steps", type=int, default=20,
+                    help="substeps of the batched-scene run of the default line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
@@ -322,6 +408,7 @@ def main():
     ctx.synchronize()
     tuning = None if args.no_autotune else ctx.autotune(3)
     start = ctx.download()
+    start_at = args.warmup
     # Whole-job throughput: substeps replayed from a captured HIP graph (the
     # capture happens during the first, untimed call), state resident in HBM,
     # no per-kernel events in the timed region.
@@ -374,7 +461,8 @@ def main():
             net.load(args.net)
         if net is not None:
             cmp = warm_train.compare(ctx, net, pos, vel, tol_params, args.tol_steps,
-                                     log=lambda m: print(m, file=sys.stderr))
+                                     log=lambda m: print(m, file=sys.stderr),
+                                     prev_scale=args.prev_scale)
         else:
             ctx.upload(pos, vel)
             ctx.set_params(tol_params)
@@ -411,9 +499,47 @@ def main():
                          "warm_speedup": (wm["particle_updates_per_s"] /
                                           pl["particle_updates_per_s"]) if wm else None,
                          "network": os.path.relpath(args.net, ROOT) if net else None}
+        pv = cmp.get("prev")
+        if pv:
+            out["jacobi"].update({
+                "prev_scale": args.prev_scale,
+                "prev_iters_per_substep": pv["iters_per_substep"],
+                "prev_mean_final_error": pv["mean_final_error"],
+                "prev_error_after_iterations": pv["error_after_iterations"],
+                "prev_fallbacks": pv["fallbacks"],
+                "prev_particle_updates_per_s": pv["particle_updates_per_s"],
+                "prev_speedup": pv["particle_updates_per_s"] / pl["particle_updates_per_s"]})
+        # equal accuracy: the fewest fixed iterations with which each mode
+        # reaches the plain solver's error at the cap, timed from the same state
+        eq = warm_train.equal_accuracy(ctx, net, pos, vel, gpu_params(sc, iters=10),
+                                       args.eq_steps, cap=10, prev_scale=args.prev_scale)
+        out["jacobi"]["equal_accuracy"] = {
+            k: v for k, v in eq.items() if k != "curves"}
+        out["jacobi"]["equal_accuracy"]["error_by_iterations"] = {
+            m: [r["mean_error"] for r in c] for m, c in eq["curves"].items()}
         ctx.upload(pos, vel)
         out["kinetic_energy_J"] = ctx.kinetic_energy()
+        # the same throughput over later windows of the run: the mean neighbour
+        # count and the longest lists grow as the columns collapse
+        ctx.set_params(gpu_params(sc))
+        out["later_windows"] = [later_window(ctx, (start_at, start), a, 20, not args.no_graph)
+                                for a in (50, 100)]
     ctx.close()
+    if not args.no_extras:
+        # the batched-scene workload, summarised (full line: --workload batch)
+        b = run_batch(args.scenes, args.batch_steps, args.warmup, graph=not args.no_graph,
+                      autotune=not args.no_autotune, net_path=args.batch_net)
+        out["batch"] = {k: b[k] for k in ("value", "unit", "ms_per_step", "steps",
+                                          "scene_max_density_error_plain", "warm_start",
+                                          "roofline")}
+        # launch gaps and tails: the graph-replayed substep against the sum of
+        # its kernels' durations (event-timed eager launches of the same substeps)
+        ksum = b["kernels_ms_per_substep_eager"]["_sum"]
+        out["batch"]["kernel_sum_ms_per_substep"] = ksum
+        out["batch"]["idle_fraction"] = max(0.0, 1.0 - ksum / b["ms_per_step"])
+        out["batch"]["workload"] = b["config"]["workload"]
+        out["batch"]["particles"] = b["config"]["particles"]
+        out["batch"]["scenes"] = b["config"]["scenes"]
     if not args.no_cpu and not args.no_extras:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_steps, threads)

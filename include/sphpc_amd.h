@@ -423,6 +423,13 @@ int sph_mlp_load(sph_mlp *net, const char *path);
  * predicted step raises the maximum density error, the substep falls back to
  * the plain first iteration (sph_stats.warm_fallback). */
 int sph_set_warm_start(sph_ctx *ctx, sph_mlp *net);
+/* The non-learned baseline of the same warm start: each following fused
+ * substep starts its solve from `scale` times the particle's accumulated
+ * multiplier of the previous substep, with the same plain iterations after it
+ * and the same fallback (the reference restarts every solve from zero,
+ * particles.c:588-666).  0 turns it off; a network warm start is turned off
+ * by it and turns it off.  Scale in [0, 4]. */
+int sph_set_warm_start_previous(sph_ctx *ctx, float scale);
 /* Record features and accumulated multipliers of every following fused
  * substep (plain solve; takes precedence over the warm start). */
 int sph_set_sample_logging(sph_ctx *ctx, int enable);

@@ -150,6 +150,7 @@ def lib():
     L.sph_mlp_save.argtypes = [vp, C.c_char_p]
     L.sph_mlp_load.argtypes = [vp, C.c_char_p]
     L.sph_set_warm_start.argtypes = [vp, vp]
+    L.sph_set_warm_start_previous.argtypes = [vp, C.c_float]
     L.sph_set_graph.argtypes = [vp, ip]
     L.sph_autotune.argtypes = [vp, ip, C.POINTER(Tuning)]
     L.sph_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
@@ -356,6 +357,11 @@ class Context:
     def set_warm_start(self, net):
         """Warm-start following fused substeps with `net` (an Mlp), None = off."""
         check(lib().sph_set_warm_start(self._h, net.handle if net is not None else None))
+
+    def set_warm_start_previous(self, scale):
+        """Warm-start following fused substeps from scale x the previous
+        substep's accumulated multiplier (0 = off)."""
+        check(lib().sph_set_warm_start_previous(self._h, float(scale)))
 
     def set_sample_logging(self, enable):
         check(lib().sph_set_sample_logging(self._h, int(bool(enable))))

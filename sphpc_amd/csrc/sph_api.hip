@@ -805,13 +805,14 @@ static int substep_launch(sph_ctx *c, bool *warm_out) {
   // Logged or warm-started solve: the first iteration's lambda pass also
   // computes the network features (and, warm, replaces lambda by the
   // network's accumulated multiplier); later iterations are the plain ones.
-  const bool warm = iters > 0 && (c->log_on || c->warm_on);
+  const bool prev = c->warm_prev_scale > 0.0f && !c->log_on;
+  const bool warm = iters > 0 && (c->log_on || c->warm_on || prev);
   *warm_out = warm;
   int it0 = 0;
   const float4 *pre_warm = nullptr;
   const bool fallback = warm && !c->log_on && iters > 1;
   if (warm) {
-    if ((rc = launch_warm_lambda(c, c->log_on))) return rc;
+    if ((rc = prev ? launch_warm_prev(c) : launch_warm_lambda(c, c->log_on))) return rc;
     // a batch decides the fallback per scene: errors before the step
     if (fallback && c->nscenes && (rc = launch_scene_errors(c, c->scene_err0))) return rc;
     pre_warm = c->pos;
@@ -924,6 +925,7 @@ extern "C" int sph_set_warm_start(sph_ctx *c, sph_mlp *net) {
     c->warm_on = false;
     return SPH_OK;
   }
+  c->warm_prev_scale = 0.0f;
   int rc = ensure_warm_buffers(c, false);
   if (rc) return rc;
   if (!c->warm_net) SPH_HIP(hipMalloc((void **)&c->warm_net, sizeof(MlpPacked)));
@@ -931,6 +933,19 @@ extern "C" int sph_set_warm_start(sph_ctx *c, sph_mlp *net) {
   if (rc) return rc;
   SPH_HIP(hipStreamSynchronize(c->stream));
   c->warm_on = true;
+  return SPH_OK;
+}
+
+extern "C" int sph_set_warm_start_previous(sph_ctx *c, float scale) {
+  if (!c) return arg_error("context is NULL");
+  if (!(scale >= 0.0f) || scale > 4.0f) return arg_error("scale must be in [0, 4]");
+  graph_reset(c);
+  if (scale > 0.0f) {
+    int rc = ensure_warm_buffers(c, false);
+    if (rc) return rc;
+    c->warm_on = false;
+  }
+  c->warm_prev_scale = scale;
   return SPH_OK;
 }
 

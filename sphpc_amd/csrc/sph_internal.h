@@ -207,6 +207,7 @@ struct sph_ctx {
   float *feat = nullptr;      // sorted feature rows (kMlpPad floats) when logging
   sph::MlpPacked *warm_net = nullptr;
   bool warm_on = false;
+  float warm_prev_scale = 0.0f;  // > 0: warm start from the previous multiplier
   bool log_on = false;
   long long feat_substep = -1;  // substep count after the last logged substep
   // batched scenes (sph_create_batch); nscenes == 0: single scene
@@ -307,6 +308,7 @@ int launch_collide_soa_f64(double *x, double *y, double *z, uint32_t n, const Sd
                            const SdfBox64 &b, hipStream_t s);
 int launch_kinetic_energy(sph_ctx *c, double *out_dev);
 int launch_warm_lambda(sph_ctx *c, bool log);
+int launch_warm_prev(sph_ctx *c);
 int launch_scatter_lambda(sph_ctx *c);
 int launch_warm_samples(sph_ctx *c, float *x, float *t, uint32_t count, int every);
 int launch_warm_fallback(sph_ctx *c, const float4 *pre_warm);

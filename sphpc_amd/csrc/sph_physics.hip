@@ -63,6 +63,12 @@ __device__ __forceinline__ NbList nb_list(const uint32_t *list, const uint4 *hea
   return r;
 }
 
+// Launch bounds of the delta-p kernels: at up to 16 gathers in flight at most
+// 128 VGPRs, i.e. 4 waves per SIMD instead of the 3 that the 139 VGPRs the
+// compiler takes otherwise leave (a few registers spill; measured 7% faster at
+// 8M particles; the colour and confinement passes were slower with it).
+#define GATHER_BOUNDS(NB) __launch_bounds__(kBlock, (NB) <= 16 ? 4 : 1)
+
 // Visits the c neighbours of a particle in list order, NB at a time: first the
 // batch's list quads, then its NB dependent gathers, then the arithmetic.  The
 // gathers are latency-bound, so the depth NB (gathers in flight per wave) sets
@@ -466,35 +472,17 @@ __global__ __launch_bounds__(kBlock) void k_build_lists(
     }
     // the format is per wavefront (the lanes' quads interleave)
     wide = __ballot(direct) != 0ull;
-    // Distances from the particle to the faces of its own cell: a particle of
-    // the cell below (above) along an axis lies at least lo (hi) away along
-    // it, since its coordinate is below (at or above) that face.  (A particle
-    // hashed onto the grid's edge from outside lies farther still, and a NaN
-    // distance skips nothing.)  The products are exact, each distance is
-    // rounded once; a cell is skipped only when its squared distance reaches
-    // the upper edge of the guard band, so no particle the exact test would
-    // accept is ever skipped and the lists stay those of the 27-cell scan.
-    const float lx = fmaf(-(float)cx, P.h, p.x), hx = fmaf((float)(cx + 1), P.h, -p.x);
-    const float ly = fmaf(-(float)cy, P.h, p.y), hy = fmaf((float)(cy + 1), P.h, -p.y);
-    const float lz = fmaf(-(float)cz, P.h, p.z), hz = fmaf((float)(cz + 1), P.h, -p.z);
-    const float lx2 = lx * lx, hx2 = hx * hx;
     int cl[3] = {0, 0, 0};
     for (int dz = -1; dz <= 1; ++dz) {
       const int z = (int)cz + dz;
       if (z >= 0 && z < (int)G.nz) {
         const uint32_t lb = dz < 0 ? base[0] : (dz == 0 ? base[1] : base[2]);
-        const float dz2 = dz < 0 ? lz * lz : (dz > 0 ? hz * hz : 0.0f);
         for (int dy = -1; dy <= 1; ++dy) {
           const int y = (int)cy + dy;
           if (y < 0 || y >= (int)G.ny) continue;
-          const float d2 = dz2 + (dy < 0 ? ly * ly : (dy > 0 ? hy * hy : 0.0f));
-          if (d2 >= P.h2_hi) continue;  // no cell of this row is within h
           const uint32_t row = G.base + (uint32_t)z * G.nxy + (uint32_t)y * G.nx;
-          // the row's first (last) cell only when it can hold a neighbour
-          const uint32_t xa = (x0 < cx && d2 + lx2 >= P.h2_hi) ? cx : x0;
-          const uint32_t xb = (x1 > cx && d2 + hx2 >= P.h2_hi) ? cx : x1;
-          const uint32_t j0 = cstart[row + xa];
-          const uint32_t j1 = cstart[row + xb + 1];
+          const uint32_t j0 = cstart[row + x0];
+          const uint32_t j1 = cstart[row + x1 + 1];
           // candidates 32 at a time: test all (8 loads in flight per group),
           // then append the hits of the chunk in j order
           for (uint32_t cb = j0; cb < j1; cb += 32) {
@@ -766,7 +754,7 @@ __device__ __forceinline__ float3 dp_sum(const float4 &p, const float4 *pos, con
 // With xnext, also the position update of k_position_stars from the same dp
 // (x* + dp, clamp, collision, rho into w), into another buffer.
 template <bool BATCH, int NB>
-__global__ __launch_bounds__(kBlock) void k_dp(uint32_t n, uint32_t nblocks,
+__global__ GATHER_BOUNDS(NB) void k_dp(uint32_t n, uint32_t nblocks,
                                                const float4 *__restrict__ pos,
                                                const uint32_t *__restrict__ list,
                                                const uint4 *__restrict__ head, DevParams P,
@@ -812,7 +800,7 @@ __global__ __launch_bounds__(kBlock) void k_position_stars(uint32_t n, float4 *_
 // within tolerance, the particles are copied unchanged instead.  With lam_acc,
 // adds stiff * lambda_i to the particle's accumulated multiplier.
 template <bool COLLIDE, bool BATCH, int NB>
-__global__ __launch_bounds__(kBlock) void k_dp_update(
+__global__ GATHER_BOUNDS(NB) void k_dp_update(
     uint32_t n, uint32_t nblocks, const float4 *__restrict__ pin, float4 *__restrict__ pout,
     const float *__restrict__ rho, const uint32_t *__restrict__ list,
     const uint4 *__restrict__ head, DevParams P, float scale, SdfDev o, uint32_t *stats,
@@ -961,6 +949,23 @@ __global__ __launch_bounds__(kBlock) void k_warm_lambda(
     pos[s].w = lam;
     lam_acc[s] = 0.0f;
   }
+}
+
+// Warm start from the previous substep's multiplier (no network), after a
+// plain density + lambda pass (rho, pos.w = lambda_0): keeps (lambda_0, rho)
+// for the plain fallback and starts the solve from `scale` times the
+// particle's accumulated multiplier of the previous substep (id order).
+__global__ __launch_bounds__(kBlock) void k_warm_prev(uint32_t n, float4 *__restrict__ pos,
+                                                      const float *__restrict__ rho,
+                                                      const uint32_t *__restrict__ id_s,
+                                                      const float *__restrict__ lam_prev,
+                                                      float scale, float2 *__restrict__ plain,
+                                                      float *__restrict__ lam_acc) {
+  const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= n) return;
+  plain[s] = make_float2(pos[s].w, rho[s]);
+  pos[s].w = scale * lam_prev[id_s[s]];
+  lam_acc[s] = 0.0f;
 }
 
 // Plain fallback of a warm start, after the density pass of iteration 1: if
@@ -1726,6 +1731,19 @@ int launch_warm_lambda(sph_ctx *c, bool log) {
     if (log) WARM(true, false); else WARM(false, false);
   }
 #undef WARM
+  SPH_HIP(hipGetLastError());
+  return SPH_OK;
+}
+
+int launch_warm_prev(sph_ctx *c) {
+  const uint32_t n = c->n;
+  if (!n) return SPH_OK;
+  int rc = launch_density(c, 0, 0);
+  if (rc) return rc;
+  LAUNCH_TIMED(c, K_AUX, 0,
+               k_warm_prev<<<blocks_for(n), kBlock, 0, c->stream>>>(
+                   n, c->pos, c->rho, c->id_s, c->lam_prev, c->warm_prev_scale,
+                   reinterpret_cast<float2 *>(c->dpbuf), c->lam_acc));
   SPH_HIP(hipGetLastError());
   return SPH_OK;
 }

@@ -212,17 +212,26 @@ def _offset(buf, nbytes):
     return ctypes.c_void_p(buf.ptr.value + int(nbytes))
 
 
-def compare(ctx, net, pos, vel, params, substeps, log=print):
+def _set_mode(ctx, mode, net, prev_scale):
+    ctx.set_warm_start(net if mode == "warm" else None)
+    ctx.set_warm_start_previous(prev_scale if mode == "prev" else 0.0)
+
+
+def compare(ctx, net, pos, vel, params, substeps, log=print, prev_scale=None):
     """Plain vs warm-started tolerance solve from the same state: each mode
     runs one untimed substep (which gives the warm start its previous
     multiplier), then `substeps` substeps, each synchronised and timed.
     Reports iterations, the max density error at the end of each solve, the
-    mean error after each number of iterations, and warm-start fallbacks."""
+    mean error after each number of iterations, and warm-start fallbacks.
+    Modes: plain, warm (the network `net`, if given) and prev (the previous
+    substep's multiplier times `prev_scale`, if given)."""
     out = {}
-    for mode in ("plain", "warm"):
+    modes = ["plain"] + (["warm"] if net is not None else []) + \
+        (["prev"] if prev_scale else [])
+    for mode in modes:
         ctx.upload(pos, vel)
         ctx.set_params(params)
-        ctx.set_warm_start(net if mode == "warm" else None)
+        _set_mode(ctx, mode, net, prev_scale)
         ctx.step(1)
         ctx.synchronize()
         its, errs, conv, curves, falls = [], [], [], [], 0
@@ -251,6 +260,51 @@ def compare(ctx, net, pos, vel, params, substeps, log=print):
                          particle_updates_per_s=ctx.count() * substeps / elapsed)
         log(f"[warm_train] {mode}: {out[mode]}")
     ctx.set_warm_start(None)
+    ctx.set_warm_start_previous(0.0)
+    return out
+
+
+def equal_accuracy(ctx, net, pos, vel, params, substeps, cap=10, prev_scale=None):
+    """Iterations each mode needs to reach the plain solver's error at the cap.
+    From the same state, each mode runs `substeps` substeps (after one untimed
+    one) with a fixed iteration count k = 1..cap; the target is the plain
+    solver's mean max density error at k = cap, and a mode's answer is the
+    smallest k whose mean error is within 0.1% of it, with that run's time."""
+    modes = ["plain"] + (["warm"] if net is not None else []) + \
+        (["prev"] if prev_scale else [])
+    runs = {m: [] for m in modes}
+    for mode in modes:
+        for k in range(1, cap + 1):
+            p = type(params).from_buffer_copy(params)
+            p.solve_iterations = k
+            p.density_tolerance = 0.0
+            ctx.upload(pos, vel)
+            ctx.set_params(p)
+            _set_mode(ctx, mode, net, prev_scale)
+            ctx.step(1)
+            ctx.synchronize()
+            errs, falls, t = [], 0, 0.0
+            for _ in range(substeps):
+                t0 = time.perf_counter()
+                ctx.step(1)
+                ctx.synchronize()
+                t += time.perf_counter() - t0
+                st = ctx.stats()
+                errs.append(st.max_density_error)
+                falls += st.warm_fallback
+            runs[mode].append(dict(iterations=k, mean_error=float(np.mean(errs)),
+                                   ms_per_substep=1e3 * t / substeps, fallbacks=int(falls)))
+    ctx.set_warm_start(None)
+    ctx.set_warm_start_previous(0.0)
+    target = runs["plain"][-1]["mean_error"]
+    out = {"target_error": target, "substeps": substeps, "curves": runs}
+    for mode in modes:
+        hit = [r for r in runs[mode] if r["mean_error"] <= target * 1.001]
+        out[mode] = hit[0] if hit else None
+    if out["plain"]:
+        for mode in modes[1:]:
+            out[mode + "_speedup"] = (out["plain"]["ms_per_substep"] /
+                                      out[mode]["ms_per_substep"]) if out[mode] else None
     return out
 
 
