@@ -120,10 +120,12 @@ static void derive(sph_ctx *c) {
   d.mW0 = (float)(m * W_norm * pow(h, 6.0));
   d.mWn = (float)(m * W_norm);
   {
-    const double h2f = (double)d.h2;  // h is a float, so h * h is exact in double
-    d.h2_err = (float)(h2f - h * h);
-    d.dens_k = (float)(ldexp(1.0, 26) / (h2f * h2f * h2f));
-    const double a = m * W_norm / (rho0 * (double)d.dens_k);
+    // h is a float, so h * h is exact in double
+    d.dens_k3 = (float)(pow(2.0, 26.0 / 3.0) / (h * h));
+    const double k3 = (double)d.dens_k3, c1 = k3 * h * h;
+    d.dens_c1_hi = (float)c1;
+    d.dens_c1_lo = (float)(c1 - (double)d.dens_c1_hi);
+    const double a = m * W_norm / (rho0 * k3 * k3 * k3);
     const double c0 = m * W_norm * pow(h, 6.0) / rho0 - 1.0;
     d.dens_a_hi = (float)a;
     d.dens_a_lo = (float)(a - (double)d.dens_a_hi);

@@ -582,14 +582,14 @@ __device__ __forceinline__ bool converged(const uint32_t *stats) {
   return stats[STAT_CONVERGED] != 0u;
 }
 
-// One neighbour's density term: the integer dens_k (h^2 - r^2)^3, truncated,
-// with h^2 - r^2 taken as (float(h^2) - r^2) - h2_err, i.e. without the
-// rounding of float(h^2).  Sums of at most 60 terms of at most 2^26 fit 32
-// bits; the sum is exact and independent of the order of the terms.
+// One neighbour's density term: the integer (k3 (h^2 - r^2))^3, truncated
+// (W(r) = norm (h^2 - r^2)^3, 0 for r >= h), with k3 h^2 carried as a float
+// pair so that the rounding of h^2 does not enter.  Sums of at most 60 terms
+// of at most 2^26 fit 32 bits; the sum is exact and independent of the order
+// of the terms.
 __device__ __forceinline__ uint32_t density_term(float r2, const DevParams &P) {
-  // W(r) = norm (h^2 - r^2)^3, 0 for r >= h
-  const float u = fmaxf((P.h2 - r2) - P.h2_err, 0.0f);
-  return (uint32_t)((u * u) * (u * P.dens_k));
+  const float u = fmaxf(fmaf(-r2, P.dens_k3, P.dens_c1_hi) + P.dens_c1_lo, 0.0f);
+  return (uint32_t)((u * u) * u);
 }
 
 // C = rho/rho0 - 1 from the fixed-point density sum S of cnt terms.  Each
@@ -635,7 +635,7 @@ __global__ __launch_bounds__(kBlock) void k_density_lambda(
         const float r2c = fmaxf(r2, P.rmin2);
         const float ir = fast_rsq(r2c);
         const float hr = fmaf(-r2c, ir, P.h);  // h - r, r floored at 1e-4 h
-        const float hp = fmaxf(hr, 0.0f);
+        const float hp = fmaxf(hr, 0.0f);  // a listed pair can be past h after an update
         const float g = hp * hp * ir;  // DelW(r)/r / norm
         sgx = fmaf(g, dx, sgx);
         sgy = fmaf(g, dy, sgy);
