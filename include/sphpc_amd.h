@@ -89,8 +89,10 @@ typedef struct sph_stats {
   int converged;             /* tolerance mode: reached the tolerance */
   float max_density_error;   /* last computed max_i max(rho_i-rho0,0)/rho0 */
   unsigned long long neighbor_overflow; /* hits dropped by the 60 cap (silent in
-                                           the reference, neighbors.c:285-286) */
-  unsigned long long cell_clamps;       /* cell coordinates clamped into the grid */
+                                           the reference, neighbors.c:285-286),
+                                           summed over every search the context ran */
+  unsigned long long cell_clamps;       /* cell coordinates clamped into the grid,
+                                           summed likewise */
   long long substeps;        /* substeps run on this context */
   unsigned long long neighbor_pairs;    /* sum of list lengths of the last search */
   int warm_fallback;         /* warm start reverted to the plain solve in the last substep:

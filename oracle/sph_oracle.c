@@ -682,8 +682,18 @@ void orc_set_threads(int threads) {
 #endif
 }
 
-int orc_substep(int n, orc_state *s, const orc_params *p, const orc_obstacle *o,
-                int iters, double tol, double *max_err_out) {
+/* Rounds n values to the nearest float32 (the storage of an FP32 solver). */
+static void round_f32(int n, double *a) {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; ++i) a[i] = (double)(float)a[i];
+}
+
+/* The substep of simulation.c:59-108.  With f32 set, every array the stage
+ * writes is rounded to float32 after the stage (velocity, x*, density,
+ * lambda, delta p, the colour and vorticity fields): the same algorithm with
+ * the storage of an FP32 solver, arithmetic inside each stage staying FP64. */
+static int substep_impl(int n, orc_state *s, const orc_params *p, const orc_obstacle *o,
+                        int iters, double tol, double *max_err_out, int f32) {
   const size_t N = (size_t)(n > 0 ? n : 1);
   double *dpx = (double *)malloc(sizeof(double) * N);
   double *dpy = (double *)malloc(sizeof(double) * N);
@@ -696,7 +706,9 @@ int orc_substep(int n, orc_state *s, const orc_params *p, const orc_obstacle *o,
   double *u2 = (double *)malloc(sizeof(double) * N);
 
   orc_apply_gravity(n, s->vz, p);
+  if (f32) round_f32(n, s->vz);
   orc_predict_positions(n, s->x, s->y, s->z, s->vx, s->vy, s->vz, s->xs, s->ys, s->zs, p);
+  if (f32) { round_f32(n, s->xs); round_f32(n, s->ys); round_f32(n, s->zs); }
   orc_find_all_neighbors(n, s->xs, s->ys, s->zs, p, s->nbr, s->cnt);
 
   int done = 0;
@@ -704,23 +716,30 @@ int orc_substep(int n, orc_state *s, const orc_params *p, const orc_obstacle *o,
   for (int it = 0; it <= iters; ++it) {
     if (it == iters && tol <= 0.0) break;
     orc_compute_densities(n, s->xs, s->ys, s->zs, s->nbr, s->cnt, p, s->density);
+    if (f32) round_f32(n, s->density);
     if (tol > 0.0) {
       err = orc_max_density_error(n, s->density, p->rest_density);
       if (err <= tol || it == iters) break;
     }
     orc_compute_lambda(n, s->xs, s->ys, s->zs, s->density, s->nbr, s->cnt, p, s->lambda);
+    if (f32) round_f32(n, s->lambda);
     orc_update_dps(n, s->xs, s->ys, s->zs, s->lambda, s->nbr, s->cnt, p, it, dpx, dpy, dpz);
+    if (f32) { round_f32(n, dpx); round_f32(n, dpy); round_f32(n, dpz); }
     orc_update_position_stars(n, s->xs, s->ys, s->zs, dpx, dpy, dpz, p, o);
+    if (f32) { round_f32(n, s->xs); round_f32(n, s->ys); round_f32(n, s->zs); }
     done++;
   }
   if (tol <= 0.0) err = orc_max_density_error(n, s->density, p->rest_density);
   if (max_err_out) *max_err_out = err;
 
   orc_update_velocities(n, s->xs, s->ys, s->zs, s->x, s->y, s->z, s->vx, s->vy, s->vz, p);
+  if (f32) { round_f32(n, s->vx); round_f32(n, s->vy); round_f32(n, s->vz); }
   if (p->surface_tension_on) {
     orc_compute_color(n, s->xs, s->ys, s->zs, s->density, s->nbr, s->cnt, p, t0, t1, t2);
+    if (f32) { round_f32(n, t0); round_f32(n, t1); round_f32(n, t2); }
     orc_apply_surface_force(n, s->xs, s->ys, s->zs, s->density, t0, t1, t2, s->nbr, s->cnt, p,
                             s->vx, s->vy, s->vz);
+    if (f32) { round_f32(n, s->vx); round_f32(n, s->vy); round_f32(n, s->vz); }
   }
   if (p->viscosity_on) {
     orc_apply_viscosity(n, s->xs, s->ys, s->zs, s->density, s->vx, s->vy, s->vz, s->nbr,
@@ -728,16 +747,29 @@ int orc_substep(int n, orc_state *s, const orc_params *p, const orc_obstacle *o,
     memcpy(s->vx, u0, sizeof(double) * (size_t)n);
     memcpy(s->vy, u1, sizeof(double) * (size_t)n);
     memcpy(s->vz, u2, sizeof(double) * (size_t)n);
+    if (f32) { round_f32(n, s->vx); round_f32(n, s->vy); round_f32(n, s->vz); }
   }
   if (p->vorticity_on) {
     orc_compute_vorticity(n, s->xs, s->ys, s->zs, s->density, s->vx, s->vy, s->vz, s->nbr,
                           s->cnt, p, t0, t1, t2);
+    if (f32) { round_f32(n, t0); round_f32(n, t1); round_f32(n, t2); }
     orc_apply_vorticity_confinement(n, s->xs, s->ys, s->zs, t0, t1, t2, s->nbr, s->cnt, p,
                                     s->vx, s->vy, s->vz);
+    if (f32) { round_f32(n, s->vx); round_f32(n, s->vy); round_f32(n, s->vz); }
   }
   orc_update_positions(n, s->xs, s->ys, s->zs, s->x, s->y, s->z);
 
   free(dpx); free(dpy); free(dpz); free(t0); free(t1); free(t2);
   free(u0); free(u1); free(u2);
   return done;
+}
+
+int orc_substep(int n, orc_state *s, const orc_params *p, const orc_obstacle *o,
+                int iters, double tol, double *max_err_out) {
+  return substep_impl(n, s, p, o, iters, tol, max_err_out, 0);
+}
+
+int orc_substep_f32_storage(int n, orc_state *s, const orc_params *p, const orc_obstacle *o,
+                            int iters, double tol, double *max_err_out) {
+  return substep_impl(n, s, p, o, iters, tol, max_err_out, 1);
 }

@@ -161,6 +161,11 @@ typedef struct orc_state {
 void orc_set_threads(int threads);
 int orc_substep(int n, orc_state *s, const orc_params *p, const orc_obstacle *o,
                 int iters, double tol, double *max_err_out);
+/* The same substep with every array a stage writes rounded to float32 after
+ * the stage (x*, velocities, density, lambda, delta p, colour and vorticity
+ * fields): the storage of an FP32 solver, for the long-run envelope. */
+int orc_substep_f32_storage(int n, orc_state *s, const orc_params *p, const orc_obstacle *o,
+                            int iters, double tol, double *max_err_out);
 
 #ifdef __cplusplus
 }

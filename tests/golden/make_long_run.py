@@ -7,7 +7,23 @@ statistics: maximum, 99th percentile and mean compression error of the last
 density pass,
 kinetic energy, mean neighbour count, particles at the 60-neighbour cap.
 
-Run from the repository root: python3 tests/golden/make_long_run.py"""
+The envelope of a case is the same run from initial positions nudged by one
+float32 ulp per coordinate (towards +inf, and towards -inf), the same run with
+its positions and velocities rounded to float32 after every substep, and the
+same run with every array a stage writes rounded to float32 after the stage
+(orc_substep_f32_storage: x* after the prediction and after every Jacobi
+update, density, lambda, delta p, velocities, colour and vorticity fields --
+what an FP32 solver stores): how far the statistics of the FP64 algorithm
+itself move under the smallest changes of its input and of its storage,
+checkpoint by checkpoint.  The test's bounds are set from it.
+
+Run from the repository root:
+  python3 tests/golden/make_long_run.py [case ...]             golden runs
+  python3 tests/golden/make_long_run.py --envelope [case ...]  envelope runs
+  python3 tests/golden/make_long_run.py --fp32-state case ...   its float32-state run only
+  python3 tests/golden/make_long_run.py --f32-storage case ...  its float32-storage run only
+  python3 tests/golden/make_long_run.py --out F --envelope c   write to F instead
+  python3 tests/golden/make_long_run.py --merge F              copy F's entries in"""
 import ctypes as C
 import json
 import os
@@ -78,8 +94,10 @@ def statistics(rho, v, cnt, s, rest=1000.0):
                 at_cap=int(np.sum(cnt >= orc.MAXN)))
 
 
-def run(name, case):
+def run(name, case, nudge=0, fp32_state=False, f32_storage=False):
     s, pos, box, sc = scene(case)
+    if nudge:
+        pos = np.nextafter(pos.astype(np.float32), np.float32(np.inf if nudge > 0 else -np.inf))
     L = orc.oracle()
     p = orc.make_params(s, box, dt=1.0 / 300.0, g=9.81, **case["physics"])
     ob = orc.obstacle(sc["sdf"], sc["dims"], sc["dx"], sc["wmin"], sc["wmax"]) if sc else None
@@ -91,21 +109,72 @@ def run(name, case):
     st = orc.OrcState(**{k: v.ctypes.data for k, v in a.items()})
     out = []
     t0 = time.time()
+    substep = L.orc_substep_f32_storage if f32_storage else L.orc_substep
     for i in range(case["substeps"]):
-        L.orc_substep(n, C.byref(st), C.byref(p), C.byref(ob) if ob else None, case["iters"],
-                      0.0, None)
+        substep(n, C.byref(st), C.byref(p), C.byref(ob) if ob else None, case["iters"], 0.0, None)
+        if fp32_state:
+            for k in ("x", "y", "z", "vx", "vy", "vz"):
+                a[k][:] = a[k].astype(np.float32)
         if (i + 1) % case["every"] == 0:
             v = np.stack([a["vx"], a["vy"], a["vz"]], 1)
             out.append(dict(substep=i + 1, **statistics(a["density"], v, a["cnt"], s)))
-            print(name, out[-1], f"{time.time() - t0:.0f}s", flush=True)
+            kind = "fp32_state" if fp32_state else "f32_storage" if f32_storage else nudge
+            print(name, kind, out[-1], f"{time.time() - t0:.0f}s", flush=True)
     return dict(particles=n, **{k: v for k, v in case.items()}, checkpoints=out)
 
 
-if __name__ == "__main__":
-    # optional arguments: the cases to (re)compute; the others keep their entries
-    path = os.path.join(HERE, "long_run.json")
+def _member(e):
+    for kind in ("fp32_state", "f32_storage"):
+        if e.get(kind):
+            return (kind, True)
+    return ("nudge", e.get("nudge"))
+
+
+def _save(path, update):
+    """Read-modify-write of the JSON file: update maps case -> fields.  Envelope
+    runs are merged member by member (a new run replaces the one of its kind)."""
     res = json.load(open(path)) if os.path.exists(path) else {}
-    for name in sys.argv[1:] or list(CASES):
-        res[name] = run(name, CASES[name])
-        with open(path, "w") as f:
-            json.dump(res, f, indent=1)
+    for name, fields in update.items():
+        entry = res.setdefault(name, {})
+        for key, val in fields.items():
+            if key == "envelope":
+                have = {_member(e): e for e in entry.get("envelope", [])}
+                have.update({_member(e): e for e in val})
+                entry["envelope"] = list(have.values())
+            else:
+                entry[key] = val
+    with open(path, "w") as f:
+        json.dump(res, f, indent=1)
+
+
+if __name__ == "__main__":
+    args = sys.argv[1:]
+    path = os.path.join(HERE, "long_run.json")
+    if "--out" in args:
+        k = args.index("--out")
+        path = args[k + 1]
+        del args[k:k + 2]
+    if "--merge" in args:
+        src = json.load(open(args[args.index("--merge") + 1]))
+        _save(path, src)
+        sys.exit(0)
+    envelope = "--envelope" in args
+    for flag, kind in (("--fp32-state", "fp32_state"), ("--f32-storage", "f32_storage")):
+        if flag in args:
+            # only this member of the envelope, merged into it
+            for name in [a for a in args if not a.startswith("--")]:
+                env = [{kind: True,
+                        "checkpoints": run(name, CASES[name], **{kind: True})["checkpoints"]}]
+                _save(path, {name: dict(envelope=env)})
+            sys.exit(0)
+    names = [a for a in args if not a.startswith("--")] or list(CASES)
+    for name in names:
+        if envelope:
+            env = [dict(nudge=d, checkpoints=run(name, CASES[name], d)["checkpoints"])
+                   for d in (1, -1)]
+            for kind in ("fp32_state", "f32_storage"):
+                env.append({kind: True,
+                            "checkpoints": run(name, CASES[name], **{kind: True})["checkpoints"]})
+            _save(path, {name: dict(envelope=env)})
+        else:
+            _save(path, {name: run(name, CASES[name])})

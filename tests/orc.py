@@ -110,9 +110,10 @@ def oracle():
         L.orc_max_density_error.argtypes = [C.c_int, _dp, C.c_double]
         L.orc_set_threads.restype = None
         L.orc_set_threads.argtypes = [C.c_int]
-        L.orc_substep.restype = C.c_int
-        L.orc_substep.argtypes = [C.c_int, C.POINTER(OrcState), P, C.POINTER(OrcObstacle),
-                                  C.c_int, C.c_double, C.POINTER(C.c_double)]
+        for f in ("orc_substep", "orc_substep_f32_storage"):
+            getattr(L, f).restype = C.c_int
+            getattr(L, f).argtypes = [C.c_int, C.POINTER(OrcState), P, C.POINTER(OrcObstacle),
+                                      C.c_int, C.c_double, C.POINTER(C.c_double)]
         _orc = L
     return _orc
 
