@@ -188,7 +188,7 @@ static void free_all(sph_ctx *c) {
                   c->vel_a, c->vel_b, c->aux, c->dpbuf, c->rho, c->cstage, c->lam_stage,
                   c->list, c->list_head,
                   c->stats, c->pair_partial, c->red, c->sdf_buf, c->scan.tile_sums,
-                  c->pw, c->lam_acc, c->lam_prev, c->feat, c->scenes_dev, c->scene_of,
+                  c->pw, c->sx, c->lam_acc, c->lam_prev, c->feat, c->scenes_dev, c->scene_of,
                   c->batch_sdf, c->scene_err, c->scene_err0,
                   c->warm_net};
   for (void *b : bufs)
@@ -297,6 +297,7 @@ static int ctx_allocate(sph_ctx *c, int max_particles, sph_ctx **out) {
       {(void **)&c->list, sizeof(uint32_t) * groups * kListStride},
       {(void **)&c->list_head, sizeof(uint4) * cap},
       {(void **)&c->pw, f4},
+      {(void **)&c->sx, sizeof(float) * 3 * (cap + kPosPad)},
       {(void **)&c->stats, sizeof(uint32_t) * STAT_WORDS},
       {(void **)&c->pair_partial, sizeof(uint32_t) * blocks_for((uint32_t)cap, kWave)},
       {(void **)&c->red, sizeof(double) * 4},
@@ -313,9 +314,11 @@ static int ctx_allocate(sph_ctx *c, int max_particles, sph_ctx **out) {
   if (e == hipSuccess) e = hipMemsetAsync(c->cstage, 0, sizeof(float) * cap, c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(c->pos_a, 0, f4 + sizeof(float4) * kPosPad, c->stream);
   if (e == hipSuccess) e = hipMemsetAsync(c->pos_b, 0, f4 + sizeof(float4) * kPosPad, c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(c->sx, 0, sizeof(float) * 3 * (cap + kPosPad), c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return fail(hip_fail(e, "context initialisation"));
   c->pos = c->pos_a;
+  c->soa_stride = (uint32_t)(cap + kPosPad);
   c->vel = c->vel_a;
   *out = c;
   return SPH_OK;

@@ -188,6 +188,10 @@ struct sph_ctx {
   float4 *vel_a = nullptr, *vel_b = nullptr, *vel = nullptr;  // vel: current v
   float4 *aux = nullptr;  // colour normal, then vorticity (w = |omega|)
   float4 *pw = nullptr;   // (x*, |omega|) for the confinement gather
+  // sorted x* again as three float arrays (x, then y, then z, soa_stride
+  // apart), written by the reorder pass for the list build's candidate loads
+  float *sx = nullptr;
+  uint32_t soa_stride = 0;
   float4 *dpbuf = nullptr;  // staged delta p
   float *rho = nullptr;
   // stage path: C = rho/rho0 - 1 of the last density stage (read by a lambda

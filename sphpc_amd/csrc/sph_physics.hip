@@ -394,11 +394,16 @@ __global__ __launch_bounds__(kBlock) void k_reorder(uint32_t n, const uint32_t *
                                                     const float4 *__restrict__ x_can,
                                                     float4 *__restrict__ pos,
                                                     float4 *__restrict__ xold,
-                                                    uint32_t *__restrict__ cell_end) {
+                                                    uint32_t *__restrict__ cell_end,
+                                                    float *__restrict__ sx, uint32_t stride) {
   const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
   if (s >= n) return;
   const uint32_t id = id_s[s];
-  pos[s] = xs_can[id];
+  const float4 q = xs_can[id];
+  pos[s] = q;
+  sx[s] = q.x;
+  sx[stride + s] = q.y;
+  sx[2u * stride + s] = q.z;
   xold[s] = x_can[id];
   const uint32_t k = key_s[s];
   if (s + 1 == n || key_s[s + 1] != k) cell_end[k] = s + 1;
@@ -419,6 +424,26 @@ __device__ __noinline__ bool inside_exact(float px, float py, float pz, float qx
   return r2 < h2d;
 }
 
+// Four consecutive floats from a 4-byte-aligned address (one 16-byte load).
+typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+__device__ __forceinline__ f4u ld4u(const float *p) { return *reinterpret_cast<const f4u *>(p); }
+
+// One candidate of the list build: each mask takes its compare as the next
+// bit, m = 2 m + [r2 < bound], one add-with-carry with the compare in VCC as
+// the carry (two VALU per mask, where the compiler's select, shift and or take
+// two and a half).  Candidates are fed last to first, so candidate t ends at
+// bit t.
+__device__ __forceinline__ void candidate_bits(uint32_t &lo, uint32_t &hi, float r2,
+                                               float h2_lo, float h2_hi) {
+  asm("v_cmp_gt_f32_e32 vcc, %[hl], %[r]\n\t"
+      "v_addc_co_u32_e32 %[lo], vcc, %[lo], %[lo], vcc\n\t"
+      "v_cmp_gt_f32_e32 vcc, %[hh], %[r]\n\t"
+      "v_addc_co_u32_e32 %[hi], vcc, %[hi], %[hi], vcc"
+      : [lo] "+v"(lo), [hi] "+v"(hi)
+      : [r] "v"(r2), [hl] "s"(h2_lo), [hh] "s"(h2_hi)
+      : "vcc");
+}
+
 // One thread per sorted particle.  The 3 cells (dx = -1..1) of each (dz, dy)
 // row are consecutive keys, so each row is one contiguous range of sorted
 // indices; walking rows in (dz, dy) order and indices upward reproduces the
@@ -437,6 +462,7 @@ __device__ __noinline__ bool inside_exact(float px, float py, float pz, float qx
 template <bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_build_lists(
     uint32_t n, uint32_t nblocks, const float4 *__restrict__ pos,
+    const float *__restrict__ sx, uint32_t stride,
     const uint32_t *__restrict__ key_s, const uint32_t *__restrict__ cstart, DevParams P,
     uint32_t *__restrict__ list, uint4 *__restrict__ head, uint32_t *__restrict__ stats,
     uint32_t *__restrict__ pair_partial, int force_wide, BatchDev B) {
@@ -488,20 +514,27 @@ __global__ __launch_bounds__(kBlock) void k_build_lists(
           for (uint32_t cb = j0; cb < j1; cb += 32) {
             const uint32_t len = j1 - cb;
             uint32_t lo = 0u, hi = 0u;  // bits: r2 < h2_lo, r2 < h2_hi
+            // groups last to first: a lane skips a suffix of the groups, which
+            // then come first, while its masks are still 0
 #pragma unroll
-            for (int g = 0; g < 4; ++g) {
+            for (int g = 3; g >= 0; --g) {
               if (8u * g < len) {
-                // reads up to 7 entries past j1: the position arrays are
-                // padded by kPosPad readable entries beyond the capacity
-                const float4 *Q = pos + cb + 8 * g;
-                float4 q[kBatch];
+                // 8 candidates as two 16-byte loads per coordinate array:
+                // a wave's lanes (about 8 cells) then touch 2-3 cache lines per
+                // load instead of about 8 for 16-byte records.  Reads up to 7
+                // entries past j1: each array is padded by kPosPad readable
+                // entries beyond the capacity.
+                const float *X = sx + cb + 8 * g;
+                const f4u x0 = ld4u(X), x1 = ld4u(X + 4);
+                const f4u y0 = ld4u(X + stride), y1 = ld4u(X + stride + 4);
+                const f4u z0 = ld4u(X + 2u * stride), z1 = ld4u(X + 2u * stride + 4);
+                const float qx[kBatch] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+                const float qy[kBatch] = {y0.x, y0.y, y0.z, y0.w, y1.x, y1.y, y1.z, y1.w};
+                const float qz[kBatch] = {z0.x, z0.y, z0.z, z0.w, z1.x, z1.y, z1.z, z1.w};
 #pragma unroll
-                for (int t = 0; t < kBatch; ++t) q[t] = Q[t];
-#pragma unroll
-                for (int t = 0; t < kBatch; ++t) {
-                  const float r2 = sq3(p.x - q[t].x, p.y - q[t].y, p.z - q[t].z);
-                  lo |= (r2 < P.h2_lo ? 1u : 0u) << (8 * g + t);
-                  hi |= (r2 < P.h2_hi ? 1u : 0u) << (8 * g + t);
+                for (int t = kBatch - 1; t >= 0; --t) {
+                  const float r2 = sq3(p.x - qx[t], p.y - qy[t], p.z - qz[t]);
+                  candidate_bits(lo, hi, r2, P.h2_lo, P.h2_hi);
                 }
               }
             }
@@ -1522,7 +1555,8 @@ int launch_neighbor_search(sph_ctx *c, bool fused) {
     LAUNCH_TIMED(c, K_REORDER, 0,
                  k_reorder<<<blocks_for(n), kBlock, 0, s>>>(n, c->id_s, c->key_s, c->xs_can,
                                                             c->x_can, c->pos, c->xold,
-                                                            c->cell_end));
+                                                            c->cell_end, c->sx,
+                                                            c->soa_stride));
     SPH_HIP(hipGetLastError());
   }
   // cell_start[c] = #keys < c (lower bound) = max end of the cells before c,
@@ -1535,13 +1569,15 @@ int launch_neighbor_search(sph_ctx *c, bool fused) {
     if (c->nscenes)
       LAUNCH_TIMED(c, K_BUILD_LISTS, 0,
                    k_build_lists<true><<<nb, bs, kMaxNb * bs * sizeof(uint16_t), s>>>(
-                       n, nb, c->pos, c->key_s, c->cell_start, c->dev, c->list, c->list_head,
-                       c->stats, c->pair_partial, c->tune.wide_lists ? 1 : 0, B));
+                       n, nb, c->pos, c->sx, c->soa_stride, c->key_s, c->cell_start, c->dev,
+                       c->list, c->list_head, c->stats, c->pair_partial,
+                       c->tune.wide_lists ? 1 : 0, B));
     else
       LAUNCH_TIMED(c, K_BUILD_LISTS, 0,
                    k_build_lists<false><<<nb, bs, kMaxNb * bs * sizeof(uint16_t), s>>>(
-                       n, nb, c->pos, c->key_s, c->cell_start, c->dev, c->list, c->list_head,
-                       c->stats, c->pair_partial, c->tune.wide_lists ? 1 : 0, B));
+                       n, nb, c->pos, c->sx, c->soa_stride, c->key_s, c->cell_start, c->dev,
+                       c->list, c->list_head, c->stats, c->pair_partial,
+                       c->tune.wide_lists ? 1 : 0, B));
     c->pair_blocks = nb;
     SPH_HIP(hipGetLastError());
   } else {
