@@ -1,0 +1,122 @@
+"""The batched-scene and warm-started workloads at their configured sizes.
+
+* 512 independent seeded scenes (14,296,610 particles; 1,620 to 120,150 per
+  scene, box aspect 1-4, fill 0.1-0.5, a sphere SDF on a 64^3 grid each,
+  s = 0.01 m, 4 Jacobi iterations, all terms) in one batched context, with
+  the launch autotuner, graph replay and the learned warm start on: eight
+  scenes spanning the particle counts are run as contexts of their own (same
+  network, default launch choices), and each must be bit-identical to its
+  slice of the batch after every substep, with the same maximum density error
+  and therefore the same fallback decision; the batch's fallback count is
+  bounded by the scenes that fell back among them and the scene count.
+* The 8M double dam break at a quarter of its lattice per axis (131,072
+  particles, same box, obstacle and physics) in the warm-start workload's
+  tolerance mode (Jacobi until max density error <= 0.1%, cap 10), plain,
+  warm-started by the network and warm-started from the previous substep's
+  multiplier, from the same state: every substep respects the cap, a
+  converged substep is within the tolerance, and the fallback is reported at
+  most once per substep."""
+import os
+
+import numpy as np
+import pytest
+
+from gpu_common import record
+
+pytestmark = pytest.mark.gpu
+
+sa = pytest.importorskip("sphpc_amd")
+from sphpc_amd import warm_train, workloads  # noqa: E402
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+NET = os.path.join(ROOT, "sphpc_amd", "data", "warm_net_s0.008.bin")
+
+
+def _net():
+    net = sa.Mlp()
+    net.load(NET)
+    return net
+
+
+def _batch_params(s):
+    return sa.default_params(2.0 * s, time_step=1.0 / 300.0, g=9.81, k=0.5, c=0.01,
+                             rest_density=1000.0, rest_mass=1000.0 * s ** 3,
+                             lambda_epsilon=600.0, vorticity_epsilon=0.01, vorticity_model=1,
+                             solve_iterations=4)
+
+
+def test_512_scene_batch_matches_separate_contexts():
+    s, scenes, pos = workloads.batched_scenes(512, seed=4)
+    counts = np.array([sc["particles"] for sc in scenes])
+    order = np.argsort(counts)
+    sample = sorted({int(order[int(q * (len(order) - 1))]) for q in np.linspace(0, 1, 8)})
+    p = _batch_params(s)
+    net = _net()
+    batch = sa.BatchContext(p, scenes)
+    batch.upload(pos)
+    batch.autotune(2)
+    batch.set_warm_start(net)
+    batch.set_graph(True)
+    singles = {}
+    for k in sample:
+        sc = scenes[k]
+        c = sa.Context(p, sc["size"], sc["particles"])
+        o = sc["sdf"]
+        c.set_obstacle(o["sdf"], o["dims"], o["spacing"], o["wmin"], o["wmax"])
+        a, b = batch.first[k], batch.first[k + 1]
+        c.upload(pos[a:b])
+        c.set_warm_start(net)
+        singles[k] = c
+    fell = []
+    for step in range(3):
+        batch.step(1)
+        bp, bv = batch.download()
+        errs = batch.scene_errors()
+        falls = 0
+        for k, c in singles.items():
+            c.step(1)
+            a, b = batch.first[k], batch.first[k + 1]
+            sp, sv = c.download()
+            assert np.array_equal(bp[a:b], sp), (step, k)
+            assert np.array_equal(bv[a:b], sv), (step, k)
+            assert errs[k] == c.stats().max_density_error, (step, k)
+            falls += c.stats().warm_fallback
+        total = batch.stats().warm_fallback
+        assert falls <= total <= len(scenes) - (len(sample) - falls), (step, falls, total)
+        fell.append(total)
+    record("batch512_vs_single", particles=len(pos), sampled=len(sample),
+           min_sampled=int(counts[sample].min()), max_sampled=int(counts[sample].max()),
+           fallbacks_0=fell[0], fallbacks_1=fell[1], fallbacks_2=fell[2],
+           max_scene_error=float(errs.max()))
+    for c in singles.values():
+        c.close()
+    batch.close()
+
+
+def test_quarter_scale_warm_start_tolerance_solve():
+    sc = workloads.double_dam_break(0.25)
+    n = len(sc["pos"])
+    ctx = sa.Context(warm_train.workload_params(10, 1e-3), sc["box"], n)
+    ctx.set_obstacle(sc["sdf"], sc["dims"], sc["dx"], sc["wmin"], sc["wmax"])
+    ctx.upload(sc["pos"])
+    ctx.step(30)
+    pos, vel = ctx.download()
+    res = warm_train.compare(ctx, _net(), pos, vel, warm_train.workload_params(10, 1e-3), 8,
+                             log=lambda m: None, prev_scale=1.0)
+    for mode in ("plain", "warm", "prev"):
+        r = res[mode]
+        assert r["iters_per_substep"] <= 10.0
+        assert 0 <= r["fallbacks"] <= 8
+        if r["converged_fraction"] == 1.0:
+            assert r["max_density_error"] <= 1e-3
+    assert res["plain"]["fallbacks"] == 0
+    record("quarter_scale_warm", particles=n,
+           plain_iters=res["plain"]["iters_per_substep"],
+           warm_iters=res["warm"]["iters_per_substep"],
+           plain_final_error=res["plain"]["mean_final_error"],
+           warm_final_error=res["warm"]["mean_final_error"],
+           warm_fallbacks=res["warm"]["fallbacks"],
+           prev_iters=res["prev"]["iters_per_substep"],
+           prev_final_error=res["prev"]["mean_final_error"],
+           prev_fallbacks=res["prev"]["fallbacks"])
+    ctx.close()
