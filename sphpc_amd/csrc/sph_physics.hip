@@ -499,6 +499,7 @@ __global__ __launch_bounds__(kBlock) void k_build_lists(
     // the format is per wavefront (the lanes' quads interleave)
     wide = __ballot(direct) != 0ull;
     int cl[3] = {0, 0, 0};
+    uint32_t slot = threadIdx.x;  // c * blockDim.x + threadIdx.x
     for (int dz = -1; dz <= 1; ++dz) {
       const int z = (int)cz + dz;
       if (z >= 0 && z < (int)G.nz) {
@@ -548,17 +549,41 @@ __global__ __launch_bounds__(kBlock) void k_build_lists(
               const float4 q = pos[cb + t];
               if (inside_exact(p.x, p.y, p.z, q.x, q.y, q.z, P.h2d)) lo |= 1u << t;
             }
-            while (lo) {
-              const uint32_t j = cb + (uint32_t)__builtin_ctz(lo);
-              lo &= lo - 1u;
-              if (c < kMaxNb) {
-                if (wide)
+            // append the chunk's hits in j order; the cap is tested per hit
+            // only when this chunk can reach it (`slot` = the LDS index of
+            // entry c, kept as a running sum)
+            const int nh = __builtin_popcount(lo);
+            if (c + nh <= kMaxNb) {
+              if (wide) {
+                while (lo) {
+                  const uint32_t j = cb + (uint32_t)__builtin_ctz(lo);
+                  lo &= lo - 1u;
                   out[(size_t)(c >> 2) * kQuadStride + (c & 3)] = j;
-                else
-                  sbuf[c * blockDim.x + threadIdx.x] = (uint16_t)(j - lb);
-                ++c;
+                  ++c;
+                }
               } else {
-                ++dropped;
+                while (lo) {
+                  const uint32_t j = cb + (uint32_t)__builtin_ctz(lo);
+                  lo &= lo - 1u;
+                  sbuf[slot] = (uint16_t)(j - lb);
+                  slot += blockDim.x;
+                }
+                c += nh;
+              }
+            } else {
+              while (lo) {
+                const uint32_t j = cb + (uint32_t)__builtin_ctz(lo);
+                lo &= lo - 1u;
+                if (c < kMaxNb) {
+                  if (wide)
+                    out[(size_t)(c >> 2) * kQuadStride + (c & 3)] = j;
+                  else
+                    sbuf[slot] = (uint16_t)(j - lb);
+                  slot += blockDim.x;
+                  ++c;
+                } else {
+                  ++dropped;
+                }
               }
             }
           }
