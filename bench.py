@@ -371,7 +371,7 @@ def main():
     ap.add_argument("--batch-net", default=os.path.join(ROOT, "sphpc_amd", "data",
                                                         "warm_net_s0.008.bin"),
                     help="network for the warm-started batch run ('' for none)")
-    ap.add_argument("--prev-scale", type=float, default=1.0,
+    ap.add_argument("--prev-scale", type=float, default=0.5,
                     help="scale of the previous-multiplier warm start (0: not run)")
     ap.add_argument("--eq-steps", type=int, default=5,
                     help="substeps per iteration count in the equal-accuracy comparison")
