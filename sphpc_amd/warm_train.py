@@ -1,6 +1,7 @@
 """Trains the lambda warm-start network on the GPU from the solver's own samples.
 
-    python -m sphpc_amd.warm_train --out sphpc_amd/data/warm_net_s0.008.bin
+    python -m sphpc_amd.warm_train --source dam_break --substeps 100 \
+        --out sphpc_amd/data/warm_net_s0.008.bin      (the committed network)
 
 Scenes (seeded): one fluid block per scene with sides drawn from U(0.2, 1.0) m
 on the s = 0.008 m lattice (jitter 0.01 s), at a random position in the
@@ -11,6 +12,13 @@ tolerance mode the warm start is used in (density error 1e-3, cap 10).  Every
 1e-4 (cap 100): the first gives the previous-multiplier feature, the second
 is sampled.  Samples are (12 features, C_eff) rows, C_eff = -Lambda (|grad C|^2
 + eps) with Lambda the accumulated multiplier of the converged plain solve.
+
+With `--source dam_break` the scenes are instead the double dam break itself at
+the lattice scales `--scales` (fractions of the 8M lattice per axis, same box,
+obstacle and physics), each run for `--substeps` substeps with the same
+logging; the last scale is held out (validation loss and the step-scale fit).
+The default scales 0.5 and 0.375 leave the 0.25 and 1.0 runs out of training
+and out of the fit.
 
 Training: Adam on random mini-batches with the project's HIP forward/backward
 kernels, cosine learning-rate decay; the last scenes are held out for the
@@ -75,23 +83,61 @@ def generate(ctx, xb, tb, capacity, scenes, seed, substeps, log_every, every, lo
         ctx.upload(pos)
         starts.append(rows)
         t0 = time.perf_counter()
-        iters = []
-        for k in range(1, substeps + 1):
-            sample = k % log_every == 0
-            pre = (k + 1) % log_every == 0
-            if sample or pre:
-                ctx.set_params(logged)
-                ctx.set_sample_logging(True)
-            ctx.step(1)
-            if sample:
-                st = ctx.stats()
-                iters.append(st.iterations)
-                rows += ctx.warm_samples(xb, tb, rows, capacity, every)
-            if sample or pre:
-                ctx.set_sample_logging(False)
-                ctx.set_params(plain)
+        rows, iters = _run_logged(ctx, xb, tb, rows, capacity, substeps, log_every, every,
+                                  plain, logged)
         log(f"[warm_train] scene {sc}: {len(pos)} particles, rows {rows}, "
             f"logged solve iterations {iters}, {time.perf_counter() - t0:.1f} s")
+        if rows >= capacity:
+            break
+    return rows, starts
+
+
+def _run_logged(ctx, xb, tb, rows, capacity, substeps, log_every, every, plain, logged):
+    """Runs the uploaded scene; every `log_every` substeps the substep before and
+    the substep itself are solved with logging, and the latter is sampled."""
+    iters = []
+    for k in range(1, substeps + 1):
+        sample = k % log_every == 0
+        pre = (k + 1) % log_every == 0
+        if sample or pre:
+            ctx.set_params(logged)
+            ctx.set_sample_logging(True)
+        ctx.step(1)
+        if sample:
+            iters.append(ctx.stats().iterations)
+            rows += ctx.warm_samples(xb, tb, rows, capacity, every)
+        if sample or pre:
+            ctx.set_sample_logging(False)
+            ctx.set_params(plain)
+    return rows, iters
+
+
+def dam_break_context(scale):
+    """A context holding the double dam break at `scale` of the 8M lattice per
+    axis, with its obstacle and the initial positions uploaded."""
+    from sphpc_amd import workloads
+    sc = workloads.double_dam_break(scale)
+    ctx = sa.Context(workload_params(), sc["box"], len(sc["pos"]))
+    ctx.set_obstacle(sc["sdf"], sc["dims"], sc["dx"], sc["wmin"], sc["wmax"])
+    ctx.upload(sc["pos"])
+    return ctx
+
+
+def generate_dam_break(xb, tb, capacity, scales, substeps, log_every, every, log=print):
+    """Samples of the double dam break at each lattice scale (one run each)."""
+    rows, starts = 0, []
+    plain = workload_params(10, 1e-3)
+    logged = workload_params(100, 1e-4)
+    for scale in scales:
+        ctx = dam_break_context(scale)
+        ctx.set_params(plain)
+        starts.append(rows)
+        t0 = time.perf_counter()
+        rows, iters = _run_logged(ctx, xb, tb, rows, capacity, substeps, log_every, every,
+                                  plain, logged)
+        log(f"[warm_train] dam break at scale {scale}: {ctx.count()} particles, rows {rows}, "
+            f"logged solve iterations {iters}, {time.perf_counter() - t0:.1f} s")
+        ctx.close()
         if rows >= capacity:
             break
     return rows, starts
@@ -129,19 +175,28 @@ def main():
     ap.add_argument("--summary", default=None, help="write a JSON summary here")
     ap.add_argument("--compare-substeps", type=int, default=20)
     ap.add_argument("--clip-quantile", type=float, default=0.995)
+    ap.add_argument("--source", choices=("blocks", "dam_break"), default="blocks")
+    ap.add_argument("--scales", default="0.5,0.375",
+                    help="dam-break lattice scales (the last is held out)")
     args = ap.parse_args()
 
-    cap_particles = 125 ** 3
-    ob = workload_obstacle()
-    ctx = sa.Context(workload_params(), BOX, cap_particles)
-    ctx.set_obstacle(ob["sdf"], ob["dims"], ob["dx"], ob["wmin"], ob["wmax"])
     xb = sa.DeviceBuffer(4 * sa.MLP_STRIDE * args.max_rows)
     tb = sa.DeviceBuffer(4 * args.max_rows)
     t0 = time.perf_counter()
-    rows, starts = generate(ctx, xb, tb, args.max_rows, args.scenes, args.seed, args.substeps,
-                            args.log_every, args.every)
+    if args.source == "dam_break":
+        scales = [float(v) for v in args.scales.split(",")]
+        rows, starts = generate_dam_break(xb, tb, args.max_rows, scales, args.substeps,
+                                          args.log_every, args.every)
+        hold = starts[-1] if len(starts) > 1 else rows
+        ctx = dam_break_context(scales[-1])
+    else:
+        ob = workload_obstacle()
+        ctx = sa.Context(workload_params(), BOX, 125 ** 3)
+        ctx.set_obstacle(ob["sdf"], ob["dims"], ob["dx"], ob["wmin"], ob["wmax"])
+        rows, starts = generate(ctx, xb, tb, args.max_rows, args.scenes, args.seed,
+                                args.substeps, args.log_every, args.every)
+        hold = starts[-args.holdout] if args.holdout and len(starts) > args.holdout else rows
     t_gen = time.perf_counter() - t0
-    hold = starts[-args.holdout] if args.holdout and len(starts) > args.holdout else rows
     print(f"[warm_train] {rows} rows ({hold} training) in {t_gen:.1f} s", flush=True)
 
     # Targets are heavy-tailed (particles pinned at walls accumulate large
@@ -179,7 +234,15 @@ def main():
         # them (mean max-density error at the iteration cap), then compare
         # plain and warm-started solves at the workload's tolerance.
         states = []
-        for k in range(2):
+        if args.source == "dam_break":
+            # the held-out scale's run, at two points of its course
+            ctx.set_params(workload_params(10, 1e-3))
+            ctx.set_warm_start(None)
+            ctx.step(30)
+            states.append(ctx.download())
+            ctx.step(30)
+            states.append(ctx.download())
+        for k in range(2 if args.source == "blocks" else 0):
             rng = np.random.default_rng(args.seed * 1000 + 990 + k)
             pos = sa.scene_lattice([random_block(rng, floor=True)], SPACING, jitter=0.01,
                                    seed=70 + k)
@@ -189,7 +252,7 @@ def main():
             ctx.step(30)
             states.append(ctx.download())
         fit = {}
-        for scale in (0.25, 0.5, 0.75, 1.0):
+        for scale in (0.125, 0.25, 0.5, 0.75, 1.0):
             net.set_step_scale(scale)
             errs = [compare(ctx, net, p, v, workload_params(10, 1e-3), 6,
                             log=lambda m: None)["warm"]["mean_final_error"] for p, v in states]
