@@ -504,12 +504,21 @@ __global__ __launch_bounds__(kBlock) void k_build_lists(
       const int z = (int)cz + dz;
       if (z >= 0 && z < (int)G.nz) {
         const uint32_t lb = dz < 0 ? base[0] : (dz == 0 ? base[1] : base[2]);
-        for (int dy = -1; dy <= 1; ++dy) {
-          const int y = (int)cy + dy;
-          if (y < 0 || y >= (int)G.ny) continue;
-          const uint32_t row = G.base + (uint32_t)z * G.nxy + (uint32_t)y * G.nx;
-          const uint32_t j0 = cstart[row + x0];
-          const uint32_t j1 = cstart[row + x1 + 1];
+        // the layer's three row ranges in one round trip (a row outside the
+        // grid is read at the particle's own row and left empty)
+        uint32_t r0[3], r1[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const int y = (int)cy + k - 1;
+          const bool in = y >= 0 && y < (int)G.ny;
+          const uint32_t row = G.base + (uint32_t)z * G.nxy + (in ? (uint32_t)y : cy) * G.nx;
+          r0[k] = cstart[row + x0];
+          r1[k] = cstart[row + x1 + 1];
+          if (!in) r1[k] = r0[k];
+        }
+        for (int k = 0; k < 3; ++k) {
+          const uint32_t j0 = k == 0 ? r0[0] : (k == 1 ? r0[1] : r0[2]);
+          const uint32_t j1 = k == 0 ? r1[0] : (k == 1 ? r1[1] : r1[2]);
           // candidates 32 at a time: test all (8 loads in flight per group),
           // then append the hits of the chunk in j order
           for (uint32_t cb = j0; cb < j1; cb += 32) {
