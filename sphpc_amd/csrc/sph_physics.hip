@@ -82,6 +82,19 @@ __device__ __forceinline__ NbList nb_list(const uint32_t *list, const uint4 *hea
 // has no candidates).
 constexpr int kBatch = 8;  // candidate batch of the list build
 constexpr uint32_t kQuadStride = kWave * 4;  // list words between a lane's quads
+// Hits the list build buffers in LDS per thread (16 bits each) for the
+// coalesced write-out; later hits of a compact list are stored directly.  At
+// 48 instead of 60 a 256-thread block holds 24 KB, which lets 6 waves per
+// SIMD in (the registers' limit) instead of 5.
+constexpr int kLdsHits = 48;
+static_assert(kLdsHits % 8 == 0 && kLdsHits <= kMaxNb, "whole quads in LDS");
+
+// Entry e of a lane's compact list (16-bit offset v), stored on its own: quad
+// e / 8 (kQuadStride words apart), 32-bit word (e % 8) / 2, low half first.
+__device__ __forceinline__ void store_entry16(uint32_t *out, int e, uint32_t v) {
+  uint16_t *w = reinterpret_cast<uint16_t *>(out + (size_t)(e >> 3) * kQuadStride + ((e & 7) >> 1));
+  w[e & 1] = (uint16_t)v;
+}
 
 template <int NB>
 __device__ __forceinline__ void load_wide(const NbList &nl, int k0, uint32_t self,
@@ -466,7 +479,7 @@ __global__ __launch_bounds__(kBlock) void k_build_lists(
     const uint32_t *__restrict__ key_s, const uint32_t *__restrict__ cstart, DevParams P,
     uint32_t *__restrict__ list, uint4 *__restrict__ head, uint32_t *__restrict__ stats,
     uint32_t *__restrict__ pair_partial, int force_wide, BatchDev B) {
-  extern __shared__ uint16_t sbuf[];  // [kMaxNb][blockDim.x]
+  extern __shared__ uint16_t sbuf[];  // [kLdsHits][blockDim.x]
   const uint32_t s = particle_index(nblocks);
   int c = 0;
   uint32_t dropped = 0;
@@ -558,27 +571,25 @@ __global__ __launch_bounds__(kBlock) void k_build_lists(
               const float4 q = pos[cb + t];
               if (inside_exact(p.x, p.y, p.z, q.x, q.y, q.z, P.h2d)) lo |= 1u << t;
             }
-            // append the chunk's hits in j order; the cap is tested per hit
-            // only when this chunk can reach it (`slot` = the LDS index of
-            // entry c, kept as a running sum)
+            // append the chunk's hits in j order; the cap and the end of the
+            // LDS buffer are tested per hit only when this chunk can reach
+            // them (`slot` = the LDS index of entry c, kept as a running sum)
             const int nh = __builtin_popcount(lo);
-            if (c + nh <= kMaxNb) {
-              if (wide) {
-                while (lo) {
-                  const uint32_t j = cb + (uint32_t)__builtin_ctz(lo);
-                  lo &= lo - 1u;
-                  out[(size_t)(c >> 2) * kQuadStride + (c & 3)] = j;
-                  ++c;
-                }
-              } else {
-                while (lo) {
-                  const uint32_t j = cb + (uint32_t)__builtin_ctz(lo);
-                  lo &= lo - 1u;
-                  sbuf[slot] = (uint16_t)(j - lb);
-                  slot += blockDim.x;
-                }
-                c += nh;
+            if (wide && c + nh <= kMaxNb) {
+              while (lo) {
+                const uint32_t j = cb + (uint32_t)__builtin_ctz(lo);
+                lo &= lo - 1u;
+                out[(size_t)(c >> 2) * kQuadStride + (c & 3)] = j;
+                ++c;
               }
+            } else if (!wide && c + nh <= kLdsHits) {
+              while (lo) {
+                const uint32_t j = cb + (uint32_t)__builtin_ctz(lo);
+                lo &= lo - 1u;
+                sbuf[slot] = (uint16_t)(j - lb);
+                slot += blockDim.x;
+              }
+              c += nh;
             } else {
               while (lo) {
                 const uint32_t j = cb + (uint32_t)__builtin_ctz(lo);
@@ -586,8 +597,10 @@ __global__ __launch_bounds__(kBlock) void k_build_lists(
                 if (c < kMaxNb) {
                   if (wide)
                     out[(size_t)(c >> 2) * kQuadStride + (c & 3)] = j;
-                  else
+                  else if (c < kLdsHits)
                     sbuf[slot] = (uint16_t)(j - lb);
+                  else
+                    store_entry16(out, c, j - lb);
                   slot += blockDim.x;
                   ++c;
                 } else {
@@ -603,13 +616,18 @@ __global__ __launch_bounds__(kBlock) void k_build_lists(
     head[s] = make_uint4(base[0], base[1], base[2],
                          (uint32_t)c | ((uint32_t)cl[0] << 8) | ((uint32_t)cl[1] << 16) |
                              (wide ? kHeadWide : 0u));
+    // the rest of a directly stored last quad is padding: offset 0 (readers
+    // decode whole quads, and a stale offset could index past the particles)
+    if (!wide && c > kLdsHits)
+      for (int e = c; (e & 7) != 0; ++e) store_entry16(out, e, 0u);
   }
   // coalesced write-out of compact quads: every lane of the wave walks the
-  // wave's longest list one quad (8 entries) at a time
+  // wave's longest list one quad (8 entries) at a time, up to the end of the
+  // LDS buffer (later quads were stored directly)
   int cmax = wide ? 0 : c;
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) cmax = max(cmax, __shfl_xor(cmax, o, 64));
-  for (int k = 0; k < cmax; k += 8) {
+  for (int k = 0; k < cmax && k < kLdsHits; k += 8) {
     if (k < c) {
       uint32_t w[4];
 #pragma unroll
@@ -1602,13 +1620,13 @@ int launch_neighbor_search(sph_ctx *c, bool fused) {
     const uint32_t nb = blocks_for(n, bs);
     if (c->nscenes)
       LAUNCH_TIMED(c, K_BUILD_LISTS, 0,
-                   k_build_lists<true><<<nb, bs, kMaxNb * bs * sizeof(uint16_t), s>>>(
+                   k_build_lists<true><<<nb, bs, kLdsHits * bs * sizeof(uint16_t), s>>>(
                        n, nb, c->pos, c->sx, c->soa_stride, c->key_s, c->cell_start, c->dev,
                        c->list, c->list_head, c->stats, c->pair_partial,
                        c->tune.wide_lists ? 1 : 0, B));
     else
       LAUNCH_TIMED(c, K_BUILD_LISTS, 0,
-                   k_build_lists<false><<<nb, bs, kMaxNb * bs * sizeof(uint16_t), s>>>(
+                   k_build_lists<false><<<nb, bs, kLdsHits * bs * sizeof(uint16_t), s>>>(
                        n, nb, c->pos, c->sx, c->soa_stride, c->key_s, c->cell_start, c->dev,
                        c->list, c->list_head, c->stats, c->pair_partial,
                        c->tune.wide_lists ? 1 : 0, B));
