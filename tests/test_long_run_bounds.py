@@ -1,4 +1,4 @@
-"""The long-run bounds against the oracle itself (CPU only).
+"""The long-run bounds and the golden file they come from (CPU only).
 
 * The golden FP64 run of the 32k dam break, rerun for its first 50 substeps,
   reproduces the stored checkpoint exactly (the oracle and the golden file
@@ -6,7 +6,9 @@
 * The same run with the particle mass, or gravity, changed by 1e-4 (a
   mistake planted in the oracle) falls outside the bounds the GPU is held to
   at that checkpoint: the bounds are loose enough for float32 and still tell
-  a changed algorithm apart."""
+  a changed algorithm apart.
+* Every case's envelope holds the four kinds of runs at the golden run's
+  checkpoints."""
 import ctypes as C
 import os
 import sys
@@ -64,3 +66,19 @@ def test_planted_mistake_falls_outside_bounds(bounds, mistake):
     gold, env = bounds
     bad, frac = lrb.check(NAME, SUB, _run(**mistake), gold, env)
     assert bad, frac
+
+
+@pytest.mark.parametrize("name", sorted(lrb.GOLD))
+def test_envelope_members_cover_the_checkpoints(name):
+    """Each case's envelope holds one run of each kind (one ulp up, one ulp
+    down, float32 state, float32 storage), at the golden run's checkpoints."""
+    g = lrb.GOLD[name]
+    env = g.get("envelope", [])
+    if not env:
+        pytest.skip(f"{name}: envelope runs not in the golden file yet")
+    kinds = sorted(("nudge", e["nudge"]) if "nudge" in e else
+                   ("fp32_state",) if e.get("fp32_state") else ("f32_storage",) for e in env)
+    assert kinds == [("f32_storage",), ("fp32_state",), ("nudge", -1), ("nudge", 1)], kinds
+    subs = [c["substep"] for c in g["checkpoints"]]
+    for e in env:
+        assert [c["substep"] for c in e["checkpoints"]] == subs
