@@ -36,14 +36,11 @@ the GPU's state at the start of the substep is the shared state:
 The substeps are 20-29 (the two columns collapsing and spreading) and 100-101
 (after the instability of the configuration, with the 60-neighbour cap in
 use)."""
-import time
-
-import numpy as np
 import pytest
 
 import orc
 import stage_checks as chk
-from gpu_common import Scene, d64, record, rel_err
+from gpu_common import Scene
 
 pytestmark = pytest.mark.gpu
 
@@ -70,85 +67,4 @@ def run():
 
 @pytest.mark.parametrize("substep", CHECKED)
 def test_substep_from_shared_state(run, substep):
-    ctx, sc, obs = run["ctx"], run["sc"], run["obs"]
-    t0 = time.time()
-    assert substep >= run["done"]
-    if substep > run["done"]:
-        ctx.step(substep - run["done"])
-    run["done"] = substep
-    x0, v0 = ctx.download()
-    tag = f"window8m_{substep}"
-
-    # stage by stage from the shared state
-    ctx.upload(x0, v0)
-    xs, nbr, cnt, at_cap = chk.check_search(ctx, sa, sc, x0, v0, tag)
-    worst = chk.check_jacobi(ctx, sa, sc, obs, nbr, cnt, sc.iters, tag)
-    chk.check_velocity_passes(ctx, sa, sc, nbr, cnt, tag)
-
-    # the fused substep from the same state; the run continues from it.  The
-    # oracle takes the rest of the substep from the GPU's predicted positions
-    # (their own error is checked above): list membership is decided on them,
-    # and a pair within an ulp of r = h in FP32 or FP64 alone would otherwise
-    # interact in one solve and not the other once the update moves it closer
-    # (at substep 20, 3,652 particles' neighbour counts differ between the two
-    # predictions, moving them by up to 6e-5 m).
-    ctx.upload(x0, v0)
-    ctx.step(1)
-    run["done"] = substep + 1
-    xg, vg = ctx.download()
-    del nbr
-    late = substep >= 100
-    x64, v64, lists = chk.oracle_substep_from_xstar(sc, obs, xs, x0, v0)
-    e_x = np.abs(d64(xg) - x64)
-    # x: the roundings of x* + delta p over the iterations plus the compounded
-    # stage errors (1e-5 of the displacement).  Over several iterations every
-    # axis of a particle carries the rounding of its largest coordinate (the
-    # others enter through the pair directions), so the ulp is that of max |x_a|.
-    disp = float(np.max(np.abs(x64 - np.stack(xs, 1))))
-    big = chk.ulp(np.max(np.abs(x64), 1, keepdims=True))
-    bound = 8.0 * big + 1e-5 * disp + chk.wall_slack(x64, sc.box, sc.h)
-    beyond = np.any(e_x > bound, 1)
-    extra = {}
-    if late:
-        # the chain's own response, by the oracle: the same substep from x*
-        # moved by one float32 ulp per coordinate, up or down at random
-        # (seeded), on the same lists (one direction everywhere would be
-        # nearly a translation, which leaves pair separations unchanged)
-        rng = np.random.default_rng(substep)
-        nudged = [np.nextafter(a.astype(np.float32),
-                               np.where(rng.random(len(a)) < 0.5, -np.inf, np.inf)
-                               .astype(np.float32)).astype(np.float64) for a in xs]
-        x64u, _, _ = chk.oracle_substep_from_xstar(sc, obs, nudged, x0, v0, lists=lists)
-        sens = np.any(np.abs(x64u - x64) > bound, 1)
-        extra = dict(beyond_bound=int(beyond.sum()), oracle_nudge_beyond_bound=int(sens.sum()),
-                     x_p99_abs=float(np.quantile(np.max(e_x, 1), 0.99)),
-                     x_max_abs_all=float(np.max(e_x)))
-        del x64u, nudged, sens
-        # after the instability a one-ulp change already moves a share of the
-        # particles beyond the bound (recorded): at most 1% may be
-        assert beyond.mean() <= 0.01, (int(beyond.sum()), extra)
-    else:
-        assert not np.any(beyond), (int(np.argmax(np.max(e_x - bound, 1))),
-                                    float(np.max(e_x - bound)))
-    del lists
-    within = ~beyond
-    # v: the velocity passes normwise 1e-5, plus the position bound over dt
-    # (v = (x* - x) / dt before those passes)
-    vmax = float(np.max(np.abs(v64[within])))
-    if late:
-        # XSPH and vorticity carry the velocities of the particles outside the
-        # bound into their neighbours': per particle, at most 1% outside
-        vb = np.max(bound, 1, keepdims=True) / sc.o.dt + 1e-5 * vmax
-        v_beyond = np.any(np.abs(d64(vg) - v64) > vb, 1)
-        extra.update(v_beyond_bound=int(v_beyond.sum()))
-        assert v_beyond.mean() <= 0.01, (int(v_beyond.sum()), extra)
-        e_v = rel_err(vg[~v_beyond], v64[~v_beyond])
-    else:
-        e_v = rel_err(vg, v64)
-        v_bound = 1e-5 + float(np.max(bound)) / sc.o.dt / vmax
-        assert e_v <= v_bound, (e_v, v_bound)
-        extra.update(v_bound=v_bound)
-    record(tag + "_fused", x_max_ulp=float(np.max(e_x[within] / big[within])),
-           x_max_abs=float(np.max(e_x[within])), v_normwise=e_v, **extra)
-    record(tag, seconds=time.time() - t0, at_cap=at_cap, **{"worst_" + k: v
-                                                            for k, v in worst.items()})
+    chk.substep_from_shared_state(run, sa, substep, f"window8m_{substep}", late=substep >= 100)
