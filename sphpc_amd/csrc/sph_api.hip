@@ -121,10 +121,9 @@ static void derive(sph_ctx *c) {
   d.mWn = (float)(m * W_norm);
   {
     // h is a float, so h * h is exact in double
-    d.dens_k3 = (float)(pow(2.0, 26.0 / 3.0) / (h * h));
-    const double k3 = (double)d.dens_k3, c1 = k3 * h * h;
-    d.dens_c1_hi = (float)c1;
-    d.dens_c1_lo = (float)(c1 - (double)d.dens_c1_hi);
+    d.dens_k3 = pow(2.0, 26.0 / 3.0) / (h * h);
+    const double k3 = d.dens_k3;
+    d.dens_c1 = k3 * (h * h);
     const double a = m * W_norm / (rho0 * k3 * k3 * k3);
     const double c0 = m * W_norm * pow(h, 6.0) / rho0 - 1.0;
     d.dens_a_hi = (float)a;

@@ -44,12 +44,11 @@ struct DevParams {
   float mW0;                 // m * W(0)
   float mWn;                 // m * W_norm
   // Density sum in 32-bit fixed point: each neighbour adds the truncated
-  // integer u^3 with u = k3 (h^2 - r^2) = fma(-r^2, k3, c1_hi) + c1_lo, where
-  // c1 = k3 h^2 is a float pair (so the rounding of h^2 does not enter) and
-  // k3^3 h^6 = 2^26 bounds a term.  The constraint is C = rho/rho0 - 1 =
-  // dens_a * S + dens_c0, both constants as float pairs (hi + lo).
-  float dens_k3;
-  float dens_c1_hi, dens_c1_lo;  // k3 h^2
+  // integer u^3 with u = k3 (h^2 - r^2) = fma(-r^2, k3, c1), computed in
+  // double, where k3^3 h^6 = 2^26 bounds a term.  The constraint is
+  // C = rho/rho0 - 1 = dens_a * S + dens_c0, both constants as float pairs
+  // (hi + lo).
+  double dens_k3, dens_c1;       // k3, k3 h^2
   float dens_a_hi, dens_a_lo;    // m W_norm / (rho0 k3^3)
   float dens_c0_hi, dens_c0_lo;  // m W(0) / rho0 - 1
   float lam_k2;              // (m * DelW_norm / rho0)^2

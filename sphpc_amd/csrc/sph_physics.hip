@@ -63,10 +63,11 @@ __device__ __forceinline__ NbList nb_list(const uint32_t *list, const uint4 *hea
   return r;
 }
 
-// Launch bounds of the delta-p kernels: at up to 16 gathers in flight at most
-// 128 VGPRs, i.e. 4 waves per SIMD instead of the 3 that the 139 VGPRs the
-// compiler takes otherwise leave (a few registers spill; measured 7% faster at
-// 8M particles; the colour and confinement passes were slower with it).
+// Launch bounds of the delta-p and density kernels: at up to 16 gathers in
+// flight at most 128 VGPRs, i.e. 4 waves per SIMD instead of the 3 that the
+// 130-139 VGPRs the compiler takes otherwise leave (a few registers spill;
+// delta p measured 7% faster at 8M particles; the colour and confinement
+// passes were slower with it).
 #define GATHER_BOUNDS(NB) __launch_bounds__(kBlock, (NB) <= 16 ? 4 : 1)
 
 // Visits the c neighbours of a particle in list order, NB at a time: first the
@@ -668,12 +669,15 @@ __device__ __forceinline__ bool converged(const uint32_t *stats) {
 }
 
 // One neighbour's density term: the integer (k3 (h^2 - r^2))^3, truncated
-// (W(r) = norm (h^2 - r^2)^3, 0 for r >= h), with k3 h^2 carried as a float
-// pair so that the rounding of h^2 does not enter.  Sums of at most 60 terms
-// of at most 2^26 fit 32 bits; the sum is exact and independent of the order
-// of the terms.
+// (W(r) = norm (h^2 - r^2)^3, 0 for r >= h).  u and u^3 are computed in
+// double (half the FP32 rate on this target, a few VALU slots more per
+// neighbour): in FP32 their roundings (about 4e-7 of a term, independent
+// between terms) moved C = rho/rho0 - 1 by up to 2e-7, which near rest,
+// where C itself is 1e-3..1e-2, is lambda's largest error.  Sums of at most
+// 60 terms of at most 2^26 fit 32 bits; the sum is exact and independent of
+// the order of the terms.
 __device__ __forceinline__ uint32_t density_term(float r2, const DevParams &P) {
-  const float u = fmaxf(fmaf(-r2, P.dens_k3, P.dens_c1_hi) + P.dens_c1_lo, 0.0f);
+  const double u = fmax(fma(-(double)r2, P.dens_k3, P.dens_c1), 0.0);
   return (uint32_t)((u * u) * u);
 }
 
@@ -697,7 +701,7 @@ __device__ __forceinline__ float density_constraint(uint32_t S, int cnt, const D
 // MODE 0/1 also store C there; with lam_out, lambda goes there and pos.w
 // keeps rho (density_to_w).
 template <int MODE, int NB>
-__global__ __launch_bounds__(kBlock) void k_density_lambda(
+__global__ GATHER_BOUNDS(NB) void k_density_lambda(
     uint32_t n, uint32_t nblocks, float4 *pos, const uint32_t *__restrict__ list,
     const uint4 *__restrict__ head, DevParams P, float *__restrict__ rho_out, float *c_io,
     uint32_t *stats, int err_slot, int check_converged, int density_to_w,
