@@ -64,10 +64,11 @@ __device__ __forceinline__ NbList nb_list(const uint32_t *list, const uint4 *hea
 }
 
 // Launch bounds of the delta-p and density kernels: at up to 16 gathers in
-// flight at most 128 VGPRs, i.e. 4 waves per SIMD instead of the 3 that the
-// 130-139 VGPRs the compiler takes otherwise leave (a few registers spill;
-// delta p measured 7% faster at 8M particles; the colour and confinement
-// passes were slower with it).
+// flight at most 128 VGPRs, i.e. 4 waves per SIMD.  With 64-bit gather
+// addresses they took 130-139 VGPRs (3 waves) and the bound made delta p 7%
+// faster at 8M particles despite a few spilled registers; with the buffer
+// gathers below they take 109-113 and the bound no longer binds.  Five waves
+// (96 VGPRs) would spill 164-220 bytes per lane.
 #define GATHER_BOUNDS(NB) __launch_bounds__(kBlock, (NB) <= 16 ? 4 : 1)
 
 // Visits the c neighbours of a particle in list order, NB at a time: first the
@@ -206,12 +207,41 @@ struct Pair4 {
 // a[j] addressed as a 32-bit byte offset from the array's base (which is
 // uniform): a gather is then one shift and a load with a scalar base instead
 // of 64-bit address arithmetic per lane.  Arrays stay below 4 GiB
-// (kMaxParticles).  Measured per kernel on MI355X: 2-5% faster in the surface
-// and vorticity passes, 5-13% slower in the others, which keep 64-bit
-// addresses.
+// (kMaxParticles).  The surface pass gathers this way: measured on MI355X at
+// 8M particles it was 3% slower through Records16 below, which every other
+// gather pass uses.
 __device__ __forceinline__ float4 at32(const float4 *a, uint32_t j) {
   return *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(a) + (j << 4));
 }
+
+// Gathers of 16-byte records through a buffer resource with a 16-byte stride
+// (`buffer_load ... idxen`): the texture addresser forms base + 16 j from the
+// index itself, so a gather costs no VALU (a 64-bit address costs two, the
+// zero high word and the shift-add) and holds no address registers while in
+// flight.  At 16 gathers in flight that is 32 VGPRs: the colour and
+// confinement passes go from 138-140 to 107-110 VGPRs (3 -> 4 waves per SIMD),
+// the density and delta-p passes no longer spill, and the XSPH and vorticity
+// passes can keep 16 gathers in flight instead of 8 (measured at 8M particles:
+// vorticity 0.70 -> 0.58 ms, XSPH 0.67 -> 0.60, delta p 3% and colour and
+// confinement 3% faster, density+lambda unchanged).  An index at or past
+// `records` reads zeros instead of leaving the array.
+typedef float sph_v4f __attribute__((ext_vector_type(4)));
+__device__ sph_v4f sph_struct_load_v4f32(__amdgpu_buffer_rsrc_t rsrc, int vindex, int voffset,
+                                         int soffset, int aux)
+    __asm("llvm.amdgcn.struct.ptr.buffer.load.v4f32");
+// Word 3 of a gfx9-family buffer descriptor: 32-bit data format, no swizzle.
+constexpr int kBufferWord3 = 0x00020000;
+
+struct Records16 {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ Records16(const float4 *a, uint32_t records)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<float4 *>(a), (short)16, (int)records,
+                                            kBufferWord3)) {}
+  __device__ __forceinline__ float4 operator[](uint32_t j) const {
+    const sph_v4f v = sph_struct_load_v4f32(r, (int)j, 0, 0, 0);
+    return make_float4(v.x, v.y, v.z, v.w);
+  }
+};
 
 __device__ __forceinline__ float fast_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
@@ -715,7 +745,8 @@ __global__ GATHER_BOUNDS(NB) void k_density_lambda(
     const NbList nl = nb_list(list, head, s);
     uint32_t S = 0u;
     float sgx = 0.0f, sgy = 0.0f, sgz = 0.0f, sc = 0.0f;
-    for_neighbors<float4, NB, true, true>(nl, s, [&](uint32_t j) { return pos[j]; },
+    const Records16 G(pos, n);
+    for_neighbors<float4, NB, true, true>(nl, s, [&](uint32_t j) { return G[j]; },
                                           [&](const float4 &q) {
       const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
       const float r2 = sq3(dx, dy, dz);
@@ -821,10 +852,11 @@ __device__ __forceinline__ void collide(float &x, float &y, float &z, const SdfD
 
 // UpdateDPs (particles.c:668-746): returns sum_j (lambda_i + lambda_j) DelW(r)/r d.
 template <int NB>
-__device__ __forceinline__ float3 dp_sum(const float4 &p, const float4 *pos, const NbList &nl,
-                                         uint32_t self, const DevParams &P) {
+__device__ __forceinline__ float3 dp_sum(const float4 &p, const float4 *pos, uint32_t n,
+                                         const NbList &nl, uint32_t self, const DevParams &P) {
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
-  for_neighbors<float4, NB>(nl, self, [&](uint32_t j) { return pos[j]; }, [&](const float4 &q) {
+  const Records16 G(pos, n);
+  for_neighbors<float4, NB>(nl, self, [&](uint32_t j) { return G[j]; }, [&](const float4 &q) {
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     const float r2c = fmaxf(sq3(dx, dy, dz), P.rmin2);
     const float ir = fast_rsq(r2c);
@@ -854,7 +886,7 @@ __global__ GATHER_BOUNDS(NB) void k_dp(uint32_t n, uint32_t nblocks,
   const uint32_t s = particle_index(nblocks);
   if (s >= n) return;
   const float4 p = pos[s];
-  const float3 a = dp_sum<NB>(p, pos, nb_list(list, head, s), s, P);
+  const float3 a = dp_sum<NB>(p, pos, n, nb_list(list, head, s), s, P);
   const float4 d = make_float4(scale * a.x, scale * a.y, scale * a.z, 0.0f);
   dp[s] = d;
   if (xnext) {
@@ -908,7 +940,7 @@ __global__ GATHER_BOUNDS(NB) void k_dp_update(
     return;
   }
   if (s == 0) stats[STAT_ITERS] += 1u;
-  const float3 a = dp_sum<NB>(p, pin, nb_list(list, head, s), s, P);
+  const float3 a = dp_sum<NB>(p, pin, n, nb_list(list, head, s), s, P);
   float x = fmaf(scale, a.x, p.x), y = fmaf(scale, a.y, p.y), z = fmaf(scale, a.z, p.z);
   apply_boundary(x, y, z, geo_of<BATCH>(s, P, B), P);
   if (COLLIDE) collide(x, y, z, sdf_of<BATCH>(s, o, B));
@@ -1202,7 +1234,8 @@ __global__ __launch_bounds__(kBlock) void k_color(uint32_t n, uint32_t nblocks,
   if (WITH_VELOCITY) vel[s] = velocity_of(p, xold[s], P);
   const NbList nl = nb_list(list, head, s);
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
-  for_neighbors<float4, NB>(nl, s, [&](uint32_t j) { return pos[j]; }, [&](const float4 &q) {
+  const Records16 G(pos, n);
+  for_neighbors<float4, NB>(nl, s, [&](uint32_t j) { return G[j]; }, [&](const float4 &q) {
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     const float r2c = fmaxf(sq3(dx, dy, dz), P.rmin2);
     const float u = P.h2 - r2c;
@@ -1273,7 +1306,8 @@ __global__ __launch_bounds__(kBlock) void k_xsph(uint32_t n, uint32_t nblocks,
   const float4 vi = vin[s];
   const NbList nl = nb_list(list, head, s);
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
-  auto load = [&](uint32_t j) { return Pair4{pos[j], vin[j]}; };
+  const Records16 Gp(pos, n), Gv(vin, n);
+  auto load = [&](uint32_t j) { return Pair4{Gp[j], Gv[j]}; };
   for_neighbors<Pair4, NB, false>(nl, s, load, [&](const Pair4 &qv) {
     const float4 &q = qv.a, &vq = qv.b;
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
@@ -1304,7 +1338,8 @@ __global__ __launch_bounds__(kBlock) void k_vorticity(uint32_t n, uint32_t nbloc
   const float4 vi = vel[s];
   const NbList nl = nb_list(list, head, s);
   float ox = 0.0f, oy = 0.0f, oz = 0.0f;
-  auto load = [&](uint32_t j) { return Pair4{at32(pos, j), at32(vel, j)}; };
+  const Records16 Gp(pos, n), Gv(vel, n);
+  auto load = [&](uint32_t j) { return Pair4{Gp[j], Gv[j]}; };
   for_neighbors<Pair4, NB, false>(nl, s, load, [&](const Pair4 &qv) {
     const float4 &q = qv.a, &vq = qv.b;
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
@@ -1346,7 +1381,8 @@ __global__ __launch_bounds__(kBlock) void k_confine(
   const NbList nl = nb_list(list, head, s);
   float ex = 0.0f, ey = 0.0f, ez = 0.0f;
   // only x_j and |omega_j| of the neighbour are needed
-  for_neighbors<float4, NB>(nl, s, [&](uint32_t j) { return pw[j]; }, [&](const float4 &q) {
+  const Records16 G(pw, n);
+  for_neighbors<float4, NB>(nl, s, [&](uint32_t j) { return G[j]; }, [&](const float4 &q) {
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     const float r2c = fmaxf(sq3(dx, dy, dz), P.rmin2);
     const float ir = fast_rsq(r2c);
