@@ -77,8 +77,8 @@ __device__ __forceinline__ NbList nb_list(const uint32_t *list, const uint4 *hea
 // the speed: 16 was 13-22% faster than 8 in every gather pass on MI355X; the
 // autotuner chooses 8, 16 or 32 per kernel.  With PREFETCH the list quads of
 // batch k+1 are issued right after the gathers of batch k, so a batch costs
-// one memory round trip instead of two; that pays for 16-byte gathers and not
-// for the 32-byte (Pair4) ones.  Slots past c are never used; their gathers
+// one memory round trip instead of two; that pays for 16-byte gathers, and of
+// the 32-byte (Pair4) ones only in the XSPH pass.  Slots past c are never used; their gathers
 // load a valid index (the particle itself in a wide list, the layer-2 base in
 // a compact one, which the build sets to the particle itself when that layer
 // has no candidates).
@@ -1293,8 +1293,12 @@ __global__ __launch_bounds__(kBlock) void k_surface(uint32_t n, uint32_t nblocks
 
 // ApplyViscosity (particles.c:383-459), reading v_old and writing v_new.  The
 // distance keeps the reference's typo r^2 = dx^2 + dy^2 dz^2 (particles.c:439).
+// The only two-array pass with the list prefetch: at 16 gathers in flight it
+// takes 175 VGPRs (2 waves per SIMD), bounded to 168 (3 waves, 32 bytes
+// spilled) it ran 8% faster than without the prefetch at 8M particles; the
+// surface and vorticity passes were 0-15% slower with it.
 template <int NB>
-__global__ __launch_bounds__(kBlock) void k_xsph(uint32_t n, uint32_t nblocks,
+__global__ __launch_bounds__(kBlock, (NB) <= 16 ? 3 : 1) void k_xsph(uint32_t n, uint32_t nblocks,
                                                  const float4 *__restrict__ pos,
                                                  const float4 *__restrict__ vin,
                                                  const uint32_t *__restrict__ list,
@@ -1308,7 +1312,7 @@ __global__ __launch_bounds__(kBlock) void k_xsph(uint32_t n, uint32_t nblocks,
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
   const Records16 Gp(pos, n), Gv(vin, n);
   auto load = [&](uint32_t j) { return Pair4{Gp[j], Gv[j]}; };
-  for_neighbors<Pair4, NB, false>(nl, s, load, [&](const Pair4 &qv) {
+  for_neighbors<Pair4, NB, true>(nl, s, load, [&](const Pair4 &qv) {
     const float4 &q = qv.a, &vq = qv.b;
     const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
     const float u = P.h2 - fmaf(dy * dy * dz, dz, dx * dx);
