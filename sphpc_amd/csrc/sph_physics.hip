@@ -67,8 +67,8 @@ __device__ __forceinline__ NbList nb_list(const uint32_t *list, const uint4 *hea
 // flight at most 128 VGPRs, i.e. 4 waves per SIMD.  With 64-bit gather
 // addresses they took 130-139 VGPRs (3 waves) and the bound made delta p 7%
 // faster at 8M particles despite a few spilled registers; with the buffer
-// gathers below they take 109-113 and the bound no longer binds.  Five waves
-// (96 VGPRs) would spill 164-220 bytes per lane.
+// gathers below they take 127-128 (delta p spills 20 bytes).  Five waves (96
+// VGPRs) would spill 164-220 bytes per lane.
 #define GATHER_BOUNDS(NB) __launch_bounds__(kBlock, (NB) <= 16 ? 4 : 1)
 
 // Visits the c neighbours of a particle in list order, NB at a time: first the
@@ -217,14 +217,14 @@ __device__ __forceinline__ float4 at32(const float4 *a, uint32_t j) {
 // Gathers of 16-byte records through a buffer resource with a 16-byte stride
 // (`buffer_load ... idxen`): the texture addresser forms base + 16 j from the
 // index itself, so a gather costs no VALU (a 64-bit address costs two, the
-// zero high word and the shift-add) and holds no address registers while in
-// flight.  At 16 gathers in flight that is 32 VGPRs: the colour and
-// confinement passes go from 138-140 to 107-110 VGPRs (3 -> 4 waves per SIMD),
-// the density and delta-p passes no longer spill, and the XSPH and vorticity
-// passes can keep 16 gathers in flight instead of 8 (measured at 8M particles:
-// vorticity 0.70 -> 0.58 ms, XSPH 0.67 -> 0.60, delta p 3% and colour and
-// confinement 3% faster, density+lambda unchanged).  An index at or past
-// `records` reads zeros instead of leaving the array.
+// zero high word and the shift-add) and needs no address registers.  At 16
+// gathers in flight: colour and confinement 138-140 -> 130-133 VGPRs, delta p
+// spills 20 bytes instead of 52 and density+lambda none instead of 12, XSPH at
+// 8 gathers 112 -> 90 VGPRs (5 waves per SIMD instead of 4), and the autotuner
+// now picks 16 gathers in flight for XSPH and vorticity instead of 8 (measured
+// at 8M particles: vorticity 0.70 -> 0.58 ms, XSPH 0.67 -> 0.60, delta p,
+// colour and confinement 3% faster, density+lambda unchanged).  An index at or
+// past `records` reads zeros instead of leaving the array.
 typedef float sph_v4f __attribute__((ext_vector_type(4)));
 __device__ sph_v4f sph_struct_load_v4f32(__amdgpu_buffer_rsrc_t rsrc, int vindex, int voffset,
                                          int soffset, int aux)
@@ -1008,7 +1008,8 @@ __global__ __launch_bounds__(kBlock) void k_warm_lambda(
     uint32_t S = 0u;
     float sgx = 0.0f, sgy = 0.0f, sgz = 0.0f, sc = 0.0f;
     float mx = 0.0f, my = 0.0f, mz = 0.0f, dv = 0.0f;
-    auto load = [&](uint32_t j) { return Pair4{pos[j], xold[j]}; };
+    const Records16 Gp(pos, n), Go(xold, n);
+    auto load = [&](uint32_t j) { return Pair4{Gp[j], Go[j]}; };
     for_neighbors<Pair4, NB, false>(nl, s, load, [&](const Pair4 &qq) {
       const float4 &q = qq.a, &qo = qq.b;
       const float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
