@@ -63,12 +63,14 @@ __device__ __forceinline__ NbList nb_list(const uint32_t *list, const uint4 *hea
   return r;
 }
 
-// Launch bounds of the delta-p and density kernels: at up to 16 gathers in
-// flight at most 128 VGPRs, i.e. 4 waves per SIMD.  With 64-bit gather
-// addresses they took 130-139 VGPRs (3 waves) and the bound made delta p 7%
-// faster at 8M particles despite a few spilled registers; with the buffer
-// gathers below they take 127-128 (delta p spills 20 bytes).  Five waves (96
-// VGPRs) would spill 164-220 bytes per lane.
+// Launch bounds of the delta-p, density, colour and confinement kernels: at
+// up to 16 gathers in flight at most 128 VGPRs, i.e. 4 waves per SIMD.  With
+// 64-bit gather addresses delta p took 139 VGPRs (3 waves) and the bound made
+// it 7% faster at 8M particles despite a few spilled registers (colour and
+// confinement, at 138-140, were 1-4% slower with it).  With the buffer gathers
+// below delta p spills 20 bytes under the bound, and colour and confinement
+// (130-133 VGPRs without it) 12-16 bytes, and run 2-3% faster than at 3 waves.
+// Five waves (96 VGPRs) would spill 164-220 bytes per lane.
 #define GATHER_BOUNDS(NB) __launch_bounds__(kBlock, (NB) <= 16 ? 4 : 1)
 
 // Visits the c neighbours of a particle in list order, NB at a time: first the
@@ -1212,7 +1214,7 @@ __global__ __launch_bounds__(kBlock) void k_velocity(uint32_t n, const float4 *_
 // with UpdateVelocities (own particle only).  DelWPoly(r)/r = norm (h^2-r^2)^2
 // with r floored at 1e-4 h.
 template <bool WITH_VELOCITY, int NB>
-__global__ __launch_bounds__(kBlock) void k_color(uint32_t n, uint32_t nblocks,
+__global__ GATHER_BOUNDS(NB) void k_color(uint32_t n, uint32_t nblocks,
                                                   const float4 *__restrict__ pos,
                                                   const float4 *__restrict__ xold,
                                                   const uint32_t *__restrict__ list,
@@ -1368,7 +1370,7 @@ __global__ __launch_bounds__(kBlock) void k_vorticity(uint32_t n, uint32_t nbloc
 // is set the result is also UpdatePositions (particles.c:566-586): x* and v
 // are scattered back to particle-id order.
 template <bool COMMIT, int NB>
-__global__ __launch_bounds__(kBlock) void k_confine(
+__global__ GATHER_BOUNDS(NB) void k_confine(
     uint32_t n, uint32_t nblocks, const float4 *__restrict__ pw,
     const float4 *__restrict__ omega, const uint32_t *__restrict__ list,
     const uint4 *__restrict__ head, DevParams P, const float4 *__restrict__ vin,
