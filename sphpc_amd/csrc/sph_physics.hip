@@ -88,9 +88,11 @@ constexpr int kBatch = 8;  // candidate batch of the list build
 constexpr uint32_t kQuadStride = kWave * 4;  // list words between a lane's quads
 // Hits the list build buffers in LDS per thread (16 bits each) for the
 // coalesced write-out; later hits of a compact list are stored directly.  At
-// 48 instead of 60 a 256-thread block holds 24 KB, which lets 6 waves per
-// SIMD in (the registers' limit) instead of 5.
-constexpr int kLdsHits = 48;
+// 40 a 256-thread block holds 20 KB, so LDS admits 7 waves per SIMD, which the
+// kernel's launch bound (70 VGPRs, no spill) also fits: measured at 8M
+// particles 0.855 -> 0.842 ms against 48 slots at 6 waves (60 slots, 30 KB,
+// admitted 5).
+constexpr int kLdsHits = 40;
 static_assert(kLdsHits % 8 == 0 && kLdsHits <= kMaxNb, "whole quads in LDS");
 
 // Entry e of a lane's compact list (16-bit offset v), stored on its own: quad
@@ -496,7 +498,7 @@ __device__ __forceinline__ void candidate_bits(uint32_t &lo, uint32_t &hi, float
 // The head records the layer bases (the particle itself for a layer without
 // candidates, so that padding entries decode to a valid index) and counts.
 template <bool BATCH>
-__global__ __launch_bounds__(kBlock) void k_build_lists(
+__global__ __launch_bounds__(kBlock, 7) void k_build_lists(
     uint32_t n, uint32_t nblocks, const float4 *__restrict__ pos,
     const float *__restrict__ sx, uint32_t stride,
     const uint32_t *__restrict__ key_s, const uint32_t *__restrict__ cstart, DevParams P,
