@@ -188,7 +188,7 @@ static void free_all(sph_ctx *c) {
                   c->list, c->list_head,
                   c->stats, c->pair_partial, c->red, c->sdf_buf, c->scan.tile_sums,
                   c->pw, c->sx, c->lam_acc, c->lam_prev, c->feat, c->scenes_dev, c->scene_of,
-                  c->batch_sdf, c->scene_err, c->scene_err0,
+                  c->batch_sdf, c->scene_err, c->scene_err0, c->scene_chunks,
                   c->warm_net};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
@@ -435,12 +435,22 @@ extern "C" int sph_create_batch(const sph_params *params, int nscenes, const sph
   // scene table, per-particle scene index, obstacle grids
   const size_t np = std::max<size_t>((size_t)total, 1);
   std::vector<uint16_t> owner(np, 0);
-  for (int k = 0; k < nscenes; ++k)
+  std::vector<uint4> chunks;
+  for (int k = 0; k < nscenes; ++k) {
     for (uint32_t i = first[k]; i < first[k + 1]; ++i) owner[i] = (uint16_t)k;
+    for (uint32_t i = first[k]; i < first[k + 1]; i += kSceneChunk)
+      chunks.push_back(make_uint4((uint32_t)k, i, std::min(i + kSceneChunk, first[k + 1]), 0u));
+  }
+  c->nchunks = (uint32_t)chunks.size();
   hipError_t e = hipMalloc((void **)&c->scenes_dev, sizeof(SceneDev) * nscenes);
   if (e == hipSuccess) e = hipMalloc((void **)&c->scene_of, sizeof(uint16_t) * np);
   if (e == hipSuccess) e = hipMalloc((void **)&c->scene_err, sizeof(uint32_t) * nscenes);
   if (e == hipSuccess) e = hipMalloc((void **)&c->scene_err0, sizeof(uint32_t) * nscenes);
+  if (e == hipSuccess && c->nchunks)
+    e = hipMalloc((void **)&c->scene_chunks, sizeof(uint4) * c->nchunks);
+  if (e == hipSuccess && c->nchunks)
+    e = hipMemcpyAsync(c->scene_chunks, chunks.data(), sizeof(uint4) * c->nchunks,
+                       hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess && sdf_words) e = hipMalloc((void **)&c->batch_sdf, sizeof(float) * sdf_words);
   size_t off = 0;
   for (int k = 0; k < nscenes && e == hipSuccess; ++k) {

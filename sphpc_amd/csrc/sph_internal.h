@@ -19,6 +19,8 @@ constexpr int kListStride = kWave * kMaxNb;  // list words per 64-particle group
 // 4 GiB, so that gathers address them with 32-bit byte offsets.
 constexpr uint32_t kMaxParticles = SPH_MAX_PARTICLES;
 static_assert((kMaxParticles + 64ull) * 16ull <= (1ull << 32), "32-bit byte offsets");
+// Particles per block of the per-scene error reduction of a batch.
+constexpr uint32_t kSceneChunk = 4096;
 // Neighbour-list head per particle: the first candidate index of each of the
 // three dz layers and kHeadCount | layer-0 count << 8 | layers-0..1 count << 16
 // | kHeadWide.  A compact list holds 16-bit offsets from the layer's first
@@ -222,6 +224,10 @@ struct sph_ctx {
   uint32_t *scene_err = nullptr;
   uint32_t *scene_err0 = nullptr;  // per scene: error before a warm-started step
   std::vector<uint32_t> scene_first;  // nscenes + 1 particle offsets
+  // (scene, first, end) of particle chunks of at most kSceneChunk, each within
+  // one scene (a scene's particles are one index range in both orders)
+  uint4 *scene_chunks = nullptr;
+  uint32_t nchunks = 0;
   sph::BatchDev batch() const { return sph::BatchDev{scenes_dev, scene_of}; }
   // launch choices (sph_autotune)
   struct {

@@ -1495,27 +1495,28 @@ __global__ __launch_bounds__(kBlock) void k_collide_sorted(uint32_t n, float4 *_
 }
 
 // Max density error per scene of a batch, from the last density pass (rho in
-// sorted order; a scene's particles are contiguous there).  A wave whose
-// lanes all belong to one scene issues one atomic; waves that straddle a
-// scene boundary issue one per lane.
-__global__ __launch_bounds__(kBlock) void k_scene_errors(uint32_t n, const float *__restrict__ rho,
-                                                         const uint16_t *__restrict__ scene_of,
+// sorted order; a scene's particles are one index range there).  One block
+// per chunk of at most kSceneChunk particles of one scene (the chunk table is
+// built with the batch): the block reduces its chunk and issues one atomic,
+// at most a few dozen per scene.  With one atomic per wave, as first written,
+// the ~440 waves of a 28k-particle scene serialised on its word (0.96 ms for
+// 512 scenes, 14.3M particles, against ~0.01 ms of traffic).
+__global__ __launch_bounds__(kBlock) void k_scene_errors(const uint4 *__restrict__ chunks,
+                                                         const float *__restrict__ rho,
                                                          DevParams P, uint32_t *err_bits) {
-  const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-  const bool active = s < n;
-  const uint32_t k = active ? scene_of[s] : 0xffffffffu;
-  const float e = active ? fmaxf((rho[s] - P.rest_density) * P.inv_rest_density, 0.0f) : 0.0f;
-  const uint32_t k0 = (uint32_t)__shfl(k, 0, 64);
-  if (__all(!active || k == k0)) {
-    float m = e;
+  __shared__ float wmax[kBlock / kWave];
+  const uint4 ch = chunks[blockIdx.x];  // scene, first, end
+  float m = 0.0f;
+  for (uint32_t i = ch.y + threadIdx.x; i < ch.z; i += kBlock)
+    m = fmaxf(m, (rho[i] - P.rest_density) * P.inv_rest_density);
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
-    if ((threadIdx.x & 63) == 0 && active && m > 0.0f) {
-      const uint32_t bits = __float_as_uint(m);
-      if (bits > *(volatile uint32_t *)&err_bits[k0]) atomicMax(&err_bits[k0], bits);
-    }
-  } else if (active && e > 0.0f) {
-    atomicMax(&err_bits[k], __float_as_uint(e));
+  for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float b = wmax[0];
+    for (int w = 1; w < kBlock / kWave; ++w) b = fmaxf(b, wmax[w]);
+    if (b > 0.0f) atomicMax(&err_bits[ch.x], __float_as_uint(b));
   }
 }
 
@@ -1908,9 +1909,8 @@ int launch_scene_errors(sph_ctx *c, uint32_t *dst) {
   if (!dst) dst = c->scene_err;
   k_fill_words<<<blocks_for(ns), kBlock, 0, c->stream>>>(ns, dst, 0u);
   SPH_HIP(hipGetLastError());
-  if (c->n) {
-    k_scene_errors<<<blocks_for(c->n), kBlock, 0, c->stream>>>(c->n, c->rho, c->scene_of, c->dev,
-                                                                dst);
+  if (c->n && c->nchunks) {
+    k_scene_errors<<<c->nchunks, kBlock, 0, c->stream>>>(c->scene_chunks, c->rho, c->dev, dst);
     SPH_HIP(hipGetLastError());
   }
   return SPH_OK;
