@@ -299,7 +299,7 @@ static int ctx_allocate(sph_ctx *c, int max_particles, sph_ctx **out) {
       {(void **)&c->sx, sizeof(float) * 3 * (cap + kPosPad)},
       {(void **)&c->stats, sizeof(uint32_t) * STAT_WORDS},
       {(void **)&c->pair_partial, sizeof(uint32_t) * blocks_for((uint32_t)cap, kWave)},
-      {(void **)&c->red, sizeof(double) * 4},
+      {(void **)&c->red, sizeof(double) * std::max<size_t>(4, blocks_for((uint32_t)cap))},
   };
   for (auto &a : allocs) {
     e = hipMalloc(a.p, a.bytes);
@@ -1075,8 +1075,15 @@ extern "C" int sph_compute_energy(sph_ctx *c, double *ke) {
   if (!ke) return arg_error("NULL argument");
   rc = launch_kinetic_energy(c, c->red);
   if (rc) return rc;
-  SPH_HIP(hipMemcpyAsync(ke, c->red, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  // per-block partial sums, added here in block order: the same bits on every run
+  std::vector<double> part(c->n ? blocks_for((uint32_t)c->n) : 0);
+  if (!part.empty())
+    SPH_HIP(hipMemcpyAsync(part.data(), c->red, sizeof(double) * part.size(),
+                           hipMemcpyDeviceToHost, c->stream));
   SPH_HIP(hipStreamSynchronize(c->stream));
+  double sum = 0.0;
+  for (double v : part) sum += v;
+  *ke = sum;
   return SPH_OK;
 }
 

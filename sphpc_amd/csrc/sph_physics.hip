@@ -340,6 +340,19 @@ __device__ __forceinline__ void add_counter64(uint32_t *stats, int slot, uint32_
   if (v) atomicAdd(reinterpret_cast<unsigned long long *>(stats + slot), (unsigned long long)v);
 }
 
+// The same with one atomic per wavefront (its lanes' sum); every lane of the
+// wave must call it.  The list build's drop count is nonzero for every
+// particle at the 60-neighbour cap (tens of thousands in a late dam-break
+// substep), and per-lane atomics on one word serialise.
+__device__ __forceinline__ void add_counter64_wave(uint32_t *stats, int slot, uint32_t v) {
+  if (__ballot(v != 0u) == 0ull) return;
+  uint32_t t = v;
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) t += __shfl_xor(t, o, 64);
+  if ((threadIdx.x & 63) == 0)
+    atomicAdd(reinterpret_cast<unsigned long long *>(stats + slot), (unsigned long long)t);
+}
+
 // Block sum of v written to partial[blockIdx.x] (no atomics; the host adds the
 // partials when it reads the statistics).  All threads of the block call it.
 __device__ __forceinline__ void block_sum_to(uint32_t *partial, uint32_t v) {
@@ -666,7 +679,7 @@ __global__ __launch_bounds__(kBlock, 7) void k_build_lists(
           make_uint4(w[0], w[1], w[2], w[3]);
     }
   }
-  add_counter64(stats, STAT_DROPPED, dropped);
+  add_counter64_wave(stats, STAT_DROPPED, dropped);
   block_sum_to(pair_partial, (uint32_t)c);
 }
 
@@ -1525,7 +1538,9 @@ __global__ __launch_bounds__(kBlock) void k_fill_words(uint32_t n, uint32_t *p, 
   if (i < n) p[i] = v;
 }
 
-// FP64 kinetic energy: per-block partial sums, then one atomic per block.
+// FP64 kinetic energy: one partial sum per block into out[blockIdx.x] (the
+// caller adds them in block order, so the result does not depend on the order
+// in which blocks finish, as it did with one atomic per block).
 __global__ __launch_bounds__(kBlock) void k_kinetic_energy(uint32_t n,
                                                            const float4 *__restrict__ v,
                                                            double half_m, double *out) {
@@ -1543,11 +1558,9 @@ __global__ __launch_bounds__(kBlock) void k_kinetic_energy(uint32_t n,
   if (threadIdx.x == 0) {
     double t = 0.0;
     for (int w = 0; w < kBlock / 64; ++w) t += part[w];
-    atomicAdd(out, t);
+    out[blockIdx.x] = t;  // summed in block order on the host
   }
 }
-
-__global__ void k_zero_f64(double *p) { *p = 0.0; }
 
 // ---------------------------------------------------------------------------
 // Launchers
@@ -2083,7 +2096,6 @@ int launch_collide_soa_f64(double *x, double *y, double *z, uint32_t n, const Sd
 }
 
 int launch_kinetic_energy(sph_ctx *c, double *out_dev) {
-  k_zero_f64<<<1, 1, 0, c->stream>>>(out_dev);
   if (c->n)
     k_kinetic_energy<<<blocks_for(c->n), kBlock, 0, c->stream>>>(
         c->n, c->v_can, 0.5 * c->P.rest_mass, out_dev);
